@@ -1,0 +1,162 @@
+/*
+ * spike_ext.h -- extended entry points of the MI355X SPIKE library.
+ *
+ * The reference allows "an extended entry (handle + options) without changing" the legacy
+ * ABI; it has a factor-once / solve-many handle only at the backend level
+ * (OpenMP/spike_pardiso.h:87-115, OpenMP/superlu_test.c:129-147). This header exposes:
+ *
+ *   - a solver handle over a CSR matrix that already lives in device memory (HBM), with
+ *     precision / partition / tolerance options and per-stage statistics;
+ *   - access to the intermediate SPIKE quantities (spike tips, reduced solution) so callers
+ *     can check them against the reference's reduced system (spike_matrix.c:657-740);
+ *   - a batched solve of many independent systems of uneven size in one set of launches;
+ *   - a continuous-batching request server with memory admission and resident factors;
+ *   - the deterministic synthetic generators used by the benchmarks and tests.
+ *
+ * All functions return SPIKE_SUCCESS (0) or a negative SPIKE_ERR_* code (spike_matrix.h).
+ * Device pointers are HIP device addresses; no torch or HIP types appear in the signatures
+ * (streams are passed as void*).
+ */
+#ifndef HSPIKE_SPIKE_EXT_H
+#define HSPIKE_SPIKE_EXT_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include "spike_matrix.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* value type of the caller's CSR values, RHS and solution arrays */
+#define SPIKE_DTYPE_D 0   /* double                                   */
+#define SPIKE_DTYPE_Z 1   /* complex16 (interleaved re, im doubles)   */
+
+/* arithmetic of the factorization and spike sweeps */
+#define SPIKE_PREC_FP64  0  /* plain FP64 SPIKE (FP64 refinement only if the residual misses tol) */
+#define SPIKE_PREC_MIXED 1  /* FP32 factor + spikes, FP64 residual / reduced solve / correction   */
+
+typedef struct {
+    int    precision;      /* SPIKE_PREC_FP64 or SPIKE_PREC_MIXED                         */
+    int    partitions;     /* 0 = autotune from (n, k, nrhs, dtype); otherwise fixed P     */
+    int    max_refine;     /* cap on refinement iterations (default 8)                     */
+    double tol;            /* target per-column ||b - A x|| / ||b|| (default 1e-10)        */
+    int    reduced_maxit;  /* cap on block-Jacobi sweeps of the reduced system (default 64) */
+    double reduced_tol;    /* relative update that ends the reduced iteration (1e-15)      */
+    int    sweep_waves;    /* waves (16 columns each) per sweep workgroup: 0 auto, 1, 2, 4 */
+    int    verbose;        /* 1 = stage timings on stderr                                   */
+} spike_opts_t;
+
+typedef struct {
+    int     n, ku, kl, b, P, nrhs;
+    int64_t nnz;
+    int     precision, dtype;
+    int     refine_iters;      /* refinement corrections applied in the last solve          */
+    int     reduced_iters;     /* block-Jacobi sweeps of the reduced system in the last solve */
+    double  reduced_update;    /* last relative update of the reduced iteration             */
+    double  max_rel_residual;  /* max over columns of ||b - A x|| / ||b|| (FP64, GPU)        */
+    double  ms_convert, ms_factor, ms_spikes, ms_reduced, ms_recover, ms_residual;
+    double  ms_factor_total;   /* spike_factor wall time                                     */
+    double  ms_solve_total;    /* spike_solve_dev wall time                                  */
+    size_t  device_bytes;      /* HBM held by the handle                                     */
+} spike_stats_t;
+
+typedef struct spike_handle spike_handle_t;
+
+void spike_default_opts(spike_opts_t *o);
+
+int  spike_create(spike_handle_t **h, const spike_opts_t *o);
+int  spike_destroy(spike_handle_t *h);
+int  spike_set_opts(spike_handle_t *h, const spike_opts_t *o);
+
+/* Bind a CSR matrix held in device memory; the pointers must stay valid until the next
+ * spike_set_matrix_* or spike_destroy (zero-copy, like matrix_CreateFromComponents). */
+int  spike_set_matrix_dev(spike_handle_t *h, int dtype, int n, int64_t nnz,
+                          const int *d_rowptr, const int *d_colind, const void *d_aij);
+/* Copy a host CSR to device memory owned by the handle. */
+int  spike_set_matrix_host(spike_handle_t *h, int dtype, int n, int64_t nnz,
+                           const int *rowptr, const int *colind, const void *aij);
+
+/* Stages 0-2 for the matrix: bandwidth, schedule, CSR -> block conversion, block LU of every
+ * partition, spike tips V/W, reduced-system setup.  Factors stay resident for later solves. */
+int  spike_factor(spike_handle_t *h);
+
+/* Solve for nrhs columns (device arrays, column-major, leading dims ldb/ldx >= n) with the
+ * resident factors: RHS sweeps, FP64 reduced solve, recovery, FP64 residual and refinement.
+ * Synchronous on return.  Calls spike_factor first if the matrix is not factored. */
+int  spike_solve_dev(spike_handle_t *h, int nrhs, const void *d_b, int ldb, void *d_x, int ldx);
+
+/* Host-array convenience wrapper: stages b to HBM, solves, copies x back. */
+int  spike_solve_host(spike_handle_t *h, int nrhs, const void *b, int ldb, void *x, int ldx);
+
+int  spike_get_stats(const spike_handle_t *h, spike_stats_t *s);
+
+/* Schedule of the factored matrix: P and the first row of each partition (rows[P+1]). */
+int  spike_get_schedule(const spike_handle_t *h, int *P, int *rows, int cap);
+
+/* FP64 spike tips of the factored matrix and of the last solve's RHS, in the layout of
+ * matrix_AssembleReducedSystem / block_AssembleReducedRHS (real dtype only). */
+int  spike_get_tips(const spike_handle_t *h, double *Vt, double *Vb, double *Wt, double *Wb,
+                    double *yt, double *yb);
+
+/* Reduced-system solution of the last solve, N_r x nrhs column-major in the reference order
+ * (per partition: x^t then x^b; spike_matrix.c:1254-1332). Real dtype, first chunk of RHS. */
+int  spike_get_reduced_solution(const spike_handle_t *h, double *xr);
+
+/* Set the HIP stream (hipStream_t passed as void*) the handle launches on; NULL = default. */
+int  spike_set_stream(spike_handle_t *h, void *stream);
+
+/* Name of the last error of this handle (static string). */
+const char *spike_error_string(int code);
+
+/* ---------------- batched solve of independent systems (one set of launches) ----------- */
+typedef struct {
+    int         n;
+    int64_t     nnz;
+    const int  *d_rowptr;
+    const int  *d_colind;
+    const void *d_aij;      /* dtype of the batch */
+    int         nrhs;
+    const void *d_b;        /* n x nrhs, ld = n */
+    void       *d_x;        /* n x nrhs, ld = n */
+} spike_system_t;
+
+typedef struct spike_batch spike_batch_t;
+
+int  spike_batch_create(spike_batch_t **bt, const spike_opts_t *o);
+int  spike_batch_destroy(spike_batch_t *bt);
+/* Factor and solve nsys systems; partitions of every system share the stage launches.
+ * P per system is chosen from (n, k, nrhs, dtype) only, never from the batch mix. */
+int  spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_system_t *sys);
+int  spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s,
+                           double *max_rel_residual_per_system);
+
+/* Partition count the autotuner picks for one system (pure function of its arguments). */
+int  spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int precision);
+
+/* Relative residual ||A x - b|| / ||b|| of every column (host arrays, FP64 on the GPU). */
+int  spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,
+                         const void *aij, int nrhs, const void *x, const void *b, double *rel);
+
+/* ---------------- synthetic generators (splitmix64 per row; deterministic) -------------- */
+/* Banded matrix: each in-band off-diagonal entry present with probability `fill`
+ * (fill >= 1 means all), values U[-1,1] (complex: re and im U[-1,1]); diagonal =
+ * max(d_i * sum_j |a_ij| + diag_add, diag_floor) with d_i ~ U[dom_lo, dom_hi].
+ * Pass NULL arrays to count: *nnz_out receives nnz. rowptr[n+1], colind/aij[nnz]. */
+int  spike_gen_banded(int dtype, int n, int ku, int kl, double fill, double dom_lo, double dom_hi,
+                      double diag_add, double diag_floor, uint64_t seed,
+                      int *rowptr, int *colind, void *aij, int64_t *nnz_out);
+/* Dense column-major n x nrhs values U[lo, hi] (complex: both parts). */
+int  spike_gen_dense(int dtype, int64_t count, double lo, double hi, uint64_t seed, void *out);
+
+/* ---------------- bandwidth-reducing reordering (reverse Cuthill-McKee) ------------------ */
+/* perm[new] = old.  Works on the symmetrized pattern of A.  Returns the new bandwidths. */
+int  spike_rcm_order(int n, const int *rowptr, const int *colind, int *perm, int *ku, int *kl);
+/* B = P A P^T as CSR (host arrays), rowptr_out[n+1], colind_out/aij_out[nnz]. */
+int  spike_permute_csr(int dtype, int n, const int *rowptr, const int *colind, const void *aij,
+                       const int *perm, int *rowptr_out, int *colind_out, void *aij_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,305 @@
+"""hspike_amd -- MI355X (gfx950) SPIKE solver for banded linear systems.
+
+Python mirror of the reference's C interface (srodrb/HSPIKE OpenMP/spike_interfaces.h:28-72,
+OpenMP/spike_superlu.h:107-135) over the C ABI of libhspike.so; every solve runs the HIP
+kernels of hspike_amd/csrc.  Host arrays are numpy; device arrays may be torch tensors on a
+ROCm device (only their data pointers are used).
+
+  dspike_core_host(A, B) / zspike_core_host / sspike_core_host / cspike_core_host
+  Solver            factor-once / solve-many handle (precision, partitions, tips, stats)
+  Batch             many independent systems of uneven size in one set of launches
+  gen_banded        deterministic synthetic banded matrices (splitmix64 per row)
+  rcm / permute     reverse Cuthill-McKee reordering of a general sparse matrix
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SPIKE_DTYPE_D, SPIKE_DTYPE_Z, SPIKE_PREC_FP64, SPIKE_PREC_MIXED, SpikeError, check, load
+
+__all__ = [
+    "CSRMatrix", "Solver", "Batch", "gen_banded", "gen_dense", "dspike_core_host", "dspike_core_host_blocking",
+    "zspike_core_host", "sspike_core_host", "cspike_core_host", "residual", "rcm", "permute", "autotune_partitions",
+    "SpikeError", "load",
+]
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class CSRMatrix:
+    """Square CSR matrix, int32 0-based indices (OpenMP/spike_matrix.h:39-53)."""
+
+    def __init__(self, n, rowptr, colind, aij):
+        self.n = int(n)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.int32)
+        aij = np.asarray(aij)
+        self.aij = np.ascontiguousarray(aij, dtype=np.complex128 if np.iscomplexobj(aij) else np.float64)
+
+    @property
+    def nnz(self):
+        return int(self.colind.size)
+
+    @property
+    def dtype_code(self):
+        return SPIKE_DTYPE_Z if np.iscomplexobj(self.aij) else SPIKE_DTYPE_D
+
+    def bandwidth(self):
+        ku, kl = ctypes.c_int(), ctypes.c_int()
+        check(load().matrix_ComputeBandwidth(self.n, _ptr(self.colind), _ptr(self.rowptr), ctypes.byref(ku),
+                                             ctypes.byref(kl)), "matrix_ComputeBandwidth")
+        return ku.value, kl.value
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.aij, self.colind, self.rowptr), shape=(self.n, self.n))
+
+    @classmethod
+    def from_scipy(cls, M):
+        M = M.tocsr()
+        return cls(M.shape[0], M.indptr, M.indices, M.data)
+
+
+def gen_banded(n, ku, kl=None, fill=1.0, dom=(1.05, 1.05), diag_add=0.0, diag_floor=1.0, seed=0, complex=False):
+    """Synthetic banded matrix: off-diagonals present with probability `fill`, U[-1,1];
+    diagonal max(d_r * sum|a_rj| + diag_add, diag_floor) with d_r ~ U[dom[0], dom[1]]."""
+    kl = ku if kl is None else kl
+    dt = SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D
+    L = load()
+    nnz = ctypes.c_int64()
+    check(L.spike_gen_banded(dt, n, ku, kl, fill, dom[0], dom[1], diag_add, diag_floor, seed, None, None, None,
+                             ctypes.byref(nnz)), "spike_gen_banded")
+    rowptr = np.zeros(n + 1, np.int32)
+    colind = np.zeros(max(nnz.value, 1), np.int32)
+    aij = np.zeros(max(nnz.value, 1), np.complex128 if complex else np.float64)
+    check(L.spike_gen_banded(dt, n, ku, kl, fill, dom[0], dom[1], diag_add, diag_floor, seed, _ptr(rowptr),
+                             _ptr(colind), _ptr(aij), ctypes.byref(nnz)), "spike_gen_banded")
+    return CSRMatrix(n, rowptr, colind[:nnz.value], aij[:nnz.value])
+
+
+def gen_dense(n, m, lo=-1.0, hi=1.0, seed=0, complex=False):
+    """Dense column-major (Fortran-order) n x m panel with U[lo, hi] entries."""
+    out = np.zeros((n, m), dtype=np.complex128 if complex else np.float64, order="F")
+    check(load().spike_gen_dense(SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D, n * m, lo, hi, seed, _ptr(out)),
+          "spike_gen_dense")
+    return out
+
+
+def autotune_partitions(n, ku, kl, nrhs=1, complex=False, precision=SPIKE_PREC_FP64):
+    return load().spike_autotune_partitions(n, ku, kl, nrhs, SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D, precision)
+
+
+def _prec_code(precision):
+    if precision in (SPIKE_PREC_FP64, "fp64", "double"):
+        return SPIKE_PREC_FP64
+    if precision in (SPIKE_PREC_MIXED, "mixed", "fp32+fp64"):
+        return SPIKE_PREC_MIXED
+    raise ValueError("precision must be 'fp64' or 'mixed'")
+
+
+class Solver:
+    """Factor-once / solve-many GPU SPIKE handle (include/spike_ext.h)."""
+
+    def __init__(self, precision="fp64", partitions=0, tol=1e-10, max_refine=8, reduced_maxit=64,
+                 reduced_tol=1e-15, verbose=False):
+        self._L = load()
+        self._opts = _lib.default_opts(precision=_prec_code(precision), partitions=int(partitions), tol=float(tol),
+                                       max_refine=int(max_refine), reduced_maxit=int(reduced_maxit),
+                                       reduced_tol=float(reduced_tol), verbose=int(bool(verbose)))
+        h = ctypes.c_void_p()
+        check(self._L.spike_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_create")
+        self._h = h
+        self._keep = ()
+        self.n = 0
+        self.complex = False
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.spike_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_matrix(self, A):
+        """Copy a host CSRMatrix to HBM."""
+        self.n, self.complex = A.n, A.dtype_code == SPIKE_DTYPE_Z
+        check(self._L.spike_set_matrix_host(self._h, A.dtype_code, A.n, A.nnz, _ptr(A.rowptr), _ptr(A.colind),
+                                            _ptr(A.aij)), "spike_set_matrix_host")
+
+    def set_matrix_dev(self, n, rowptr, colind, aij):
+        """Bind CSR arrays already in device memory (torch tensors; kept alive by the solver)."""
+        import torch
+        cplx = aij.dtype == torch.complex128
+        self.n, self.complex = int(n), cplx
+        self._keep = (rowptr, colind, aij)
+        check(self._L.spike_set_matrix_dev(self._h, SPIKE_DTYPE_Z if cplx else SPIKE_DTYPE_D, int(n), colind.numel(),
+                                           _ptr(rowptr), _ptr(colind), _ptr(aij)), "spike_set_matrix_dev")
+
+    def factor(self):
+        check(self._L.spike_factor(self._h), "spike_factor")
+
+    def solve(self, B, allow_unconverged=False):
+        """Solve for host panel B (n or n x m); returns X (numpy, Fortran order)."""
+        dt = np.complex128 if self.complex else np.float64
+        B = np.asfortranarray(np.asarray(B, dtype=dt).reshape(self.n, -1))
+        X = np.zeros_like(B, order="F")
+        rc = self._L.spike_solve_host(self._h, B.shape[1], _ptr(B), self.n, _ptr(X), self.n)
+        if rc != 0 and not (allow_unconverged and rc == -5):
+            check(rc, "spike_solve_host")
+        return X
+
+    def solve_dev(self, B, X, nrhs=None):
+        """Solve with device panels B, X (torch tensors, column-major n x m, i.e. shape (m, n) rows)."""
+        m = nrhs if nrhs is not None else (B.numel() // self.n)
+        check(self._L.spike_solve_dev(self._h, m, _ptr(B), self.n, _ptr(X), self.n), "spike_solve_dev")
+
+    def stats(self):
+        s = _lib.Stats()
+        check(self._L.spike_get_stats(self._h, ctypes.byref(s)), "spike_get_stats")
+        return s.as_dict()
+
+    def schedule(self):
+        P = ctypes.c_int()
+        check(self._L.spike_get_schedule(self._h, ctypes.byref(P), None, 0), "spike_get_schedule")
+        rows = np.zeros(P.value + 1, np.int32)
+        check(self._L.spike_get_schedule(self._h, ctypes.byref(P), _ptr(rows), P.value + 1), "spike_get_schedule")
+        return rows
+
+    def tips(self, nrhs):
+        """FP64 spike tips in the oracle's layout: dict name -> [P, rows, cols]."""
+        st = self.stats()
+        P, ku, kl = st["P"], st["ku"], st["kl"]
+        shapes = {"Vt": (ku, ku), "Vb": (kl, ku), "Wt": (ku, kl), "Wb": (kl, kl), "yt": (ku, nrhs), "yb": (kl, nrhs)}
+        bufs = {k: np.zeros(P * r * c + 1) for k, (r, c) in shapes.items()}
+        check(self._L.spike_get_tips(self._h, *[_ptr(bufs[k]) for k in ("Vt", "Vb", "Wt", "Wb", "yt", "yb")]),
+              "spike_get_tips")
+        return {k: bufs[k][:P * r * c].reshape(P, c, r).transpose(0, 2, 1) for k, (r, c) in shapes.items()}
+
+    def reduced_solution(self, nrhs):
+        st = self.stats()
+        Nr = st["P"] * (st["ku"] + st["kl"])
+        xr = np.zeros(Nr * nrhs)
+        check(self._L.spike_get_reduced_solution(self._h, _ptr(xr)), "spike_get_reduced_solution")
+        return xr.reshape(nrhs, Nr).T
+
+
+class Batch:
+    """Factor and solve many independent device-resident systems together (include/spike_ext.h)."""
+
+    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, verbose=False, partitions=0):
+        self._L = load()
+        self._opts = _lib.default_opts(precision=_prec_code(precision), tol=float(tol), max_refine=int(max_refine),
+                                       verbose=int(bool(verbose)), partitions=int(partitions))
+        h = ctypes.c_void_p()
+        check(self._L.spike_batch_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.spike_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_dev(self, systems, complex=False, allow_unconverged=False):
+        """systems: list of dicts with n, rowptr, colind, aij, b, x (torch device tensors), nrhs."""
+        arr = (_lib.System * len(systems))()
+        for i, s in enumerate(systems):
+            arr[i].n = s["n"]
+            arr[i].nnz = s["colind"].numel()
+            arr[i].d_rowptr = s["rowptr"].data_ptr()
+            arr[i].d_colind = s["colind"].data_ptr()
+            arr[i].d_aij = s["aij"].data_ptr()
+            arr[i].nrhs = s["nrhs"]
+            arr[i].d_b = s["b"].data_ptr()
+            arr[i].d_x = s["x"].data_ptr()
+        rc = self._L.spike_batch_solve_dev(self._h, SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D, len(systems), arr)
+        if rc != 0 and not (allow_unconverged and rc == -5):
+            check(rc, "spike_batch_solve_dev")
+        rel = np.zeros(len(systems))
+        st = _lib.Stats()
+        check(self._L.spike_batch_get_stats(self._h, ctypes.byref(st), _ptr(rel)), "spike_batch_get_stats")
+        return rel, st.as_dict()
+
+
+def _legacy(fn_name, A, B, dt_np, cast_a=None):
+    L = load()
+    B = np.asfortranarray(np.asarray(B, dtype=dt_np).reshape(A.n, -1))
+    X = np.zeros_like(B, order="F")
+    aij = np.ascontiguousarray(A.aij.astype(dt_np)) if cast_a is None else cast_a
+    rc = getattr(L, fn_name)(A.n, A.nnz, B.shape[1], _ptr(A.colind), _ptr(A.rowptr), _ptr(aij), _ptr(X), _ptr(B))
+    check(rc, fn_name)
+    return X
+
+
+def dspike_core_host(A, B):
+    """X = A^{-1} B, FP64 (OpenMP/spike_interfaces.h:37-44)."""
+    return _legacy("dspike_core_host", A, B, np.float64)
+
+
+def dspike_core_host_blocking(A, B):
+    """Same with RHS columns processed in blocks of 200 (spike_interfaces.h:65-72)."""
+    return _legacy("dspike_core_host_blocking", A, B, np.float64)
+
+
+def zspike_core_host(A, B):
+    """complex128 (spike_interfaces.h:55-62)."""
+    return _legacy("zspike_core_host", A, B, np.complex128)
+
+
+def sspike_core_host(A, B):
+    """FP32 arrays; FP32 factorization with FP64 refinement (spike_interfaces.h:28-35)."""
+    return _legacy("sspike_core_host", A, B, np.float32)
+
+
+def cspike_core_host(A, B):
+    """complex64 arrays (spike_interfaces.h:46-53)."""
+    return _legacy("cspike_core_host", A, B, np.complex64)
+
+
+def residual(A, X, B):
+    """Per-column ||A x - b|| / ||b|| computed in FP64 on the GPU."""
+    dt = np.complex128 if (A.dtype_code == SPIKE_DTYPE_Z or np.iscomplexobj(B)) else np.float64
+    X = np.asfortranarray(np.asarray(X, dtype=dt).reshape(A.n, -1))
+    B = np.asfortranarray(np.asarray(B, dtype=dt).reshape(A.n, -1))
+    aij = np.ascontiguousarray(A.aij, dtype=dt)
+    rel = np.zeros(X.shape[1])
+    check(load().spike_residual_host(SPIKE_DTYPE_Z if dt == np.complex128 else SPIKE_DTYPE_D, A.n, A.nnz,
+                                     _ptr(A.rowptr), _ptr(A.colind), _ptr(aij), X.shape[1], _ptr(X), _ptr(B),
+                                     _ptr(rel)), "spike_residual_host")
+    return rel
+
+
+def rcm(A):
+    """Reverse Cuthill-McKee order: returns (perm with perm[new] = old, ku, kl of P A P^T)."""
+    perm = np.zeros(A.n, np.int32)
+    ku, kl = ctypes.c_int(), ctypes.c_int()
+    check(load().spike_rcm_order(A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(perm), ctypes.byref(ku), ctypes.byref(kl)),
+          "spike_rcm_order")
+    return perm, ku.value, kl.value
+
+
+def permute(A, perm):
+    """P A P^T as a new CSRMatrix (rows and columns renumbered by perm[new] = old)."""
+    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    rp = np.zeros(A.n + 1, np.int32)
+    ci = np.zeros(max(A.nnz, 1), np.int32)
+    av = np.zeros(max(A.nnz, 1), A.aij.dtype)
+    check(load().spike_permute_csr(A.dtype_code, A.n, _ptr(A.rowptr), _ptr(A.colind), _ptr(A.aij), _ptr(perm),
+                                   _ptr(rp), _ptr(ci), _ptr(av)), "spike_permute_csr")
+    return CSRMatrix(A.n, rp, ci[:A.nnz], av[:A.nnz])

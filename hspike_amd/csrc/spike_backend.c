@@ -1,0 +1,128 @@
+/*
+ * spike_backend.c -- the direct-solver plugin slot (spike_hip.h) on the GPU banded solver.
+ *
+ * The reference picks one backend at compile time (OpenMP/spike_algebra.h:30-36) and drives it
+ * through CreateHandler / Configure / Factorize / SolveForRHS / ShowStatistics / Finalize
+ * (spike_superlu.c:63-318, spike_pardiso.c:17-362, spike_cuda.c:353-884).  Here the handle
+ * owns a spike_handle_t: Factorize copies the CSR to HBM and factors it once, every
+ * SolveForRHS reuses the resident factors (the reference's CUDA backend re-stages panels on
+ * every call, spike_cuda.c:604-712).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "spike_hip.h"
+#include "spike_ext.h"
+
+struct DirectSolverHander_s {
+    spike_handle_t *h;
+    integer_t max_nrhs;
+    integer_t n;
+    double factor_t, solve_t;
+    integer_t rhs_block_count, rhs_column_count;
+};
+
+static double wall(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+DirectSolverHander_t *directSolver_CreateHandler(void)
+{
+    DirectSolverHander_t *s = (DirectSolverHander_t *)calloc(1, sizeof(DirectSolverHander_t));
+    if (!s) return NULL;
+    spike_opts_t o;
+    spike_default_opts(&o);
+    const char *v = getenv("HSPIKE_PARTITIONS");
+    if (v && atoi(v) > 0) o.partitions = atoi(v);
+    if (spike_create(&s->h, &o) != SPIKE_SUCCESS) { free(s); return NULL; }
+    return s;
+}
+
+Error_t directSolver_Configure(DirectSolverHander_t *handler, const integer_t max_nrhs)
+{
+    if (!handler || max_nrhs < 0) return SPIKE_ERR_INVALID_ARG;
+    handler->max_nrhs = max_nrhs;
+    return SPIKE_SUCCESS;
+}
+
+Error_t directSolver_Factorize(DirectSolverHander_t *handler, const integer_t n, const integer_t nnz,
+                               integer_t *colind, integer_t *rowptr, double *aij)
+{
+    if (!handler || n <= 0 || nnz <= 0) return SPIKE_ERR_INVALID_ARG;   /* spike_superlu.c:127-130 */
+    double t0 = wall();
+    int e = spike_set_matrix_host(handler->h, SPIKE_DTYPE_D, n, nnz, rowptr, colind, aij);
+    if (!e) e = spike_factor(handler->h);
+    handler->n = n;
+    handler->factor_t += wall() - t0;
+    return e;
+}
+
+Error_t directSolver_SolveForRHS(DirectSolverHander_t *handler, const integer_t nrhs, double *xij, double *bij)
+{
+    if (!handler || nrhs < 0 || handler->n <= 0) return SPIKE_ERR_INVALID_ARG;
+    double t0 = wall();
+    int e = spike_solve_host(handler->h, nrhs, bij, handler->n, xij, handler->n);
+    handler->solve_t += wall() - t0;
+    handler->rhs_block_count += 1;
+    handler->rhs_column_count += nrhs;
+    return e;
+}
+
+Error_t directSolver_ShowStatistics(DirectSolverHander_t *handler)
+{
+    if (!handler) return SPIKE_ERR_INVALID_ARG;
+    spike_stats_t st;
+    spike_get_stats(handler->h, &st);
+    fprintf(stderr, "\n BACKEND: HIP banded SPIKE (gfx950)\n");
+    fprintf(stderr, "  RHS packs solved        : %d\n", handler->rhs_block_count);
+    fprintf(stderr, "  RHS vectors solved      : %d\n", handler->rhs_column_count);
+    fprintf(stderr, "  partitions / block size : %d / %d\n", st.P, st.b);
+    fprintf(stderr, "  factorization time      : %.6f s\n", handler->factor_t);
+    fprintf(stderr, "  solution time           : %.6f s\n", handler->solve_t);
+    fprintf(stderr, "  last max rel. residual  : %.3e\n", st.max_rel_residual);
+    return SPIKE_SUCCESS;
+}
+
+Error_t directSolver_Finalize(DirectSolverHander_t *handler)
+{
+    if (!handler) return SPIKE_ERR_INVALID_ARG;
+    spike_destroy(handler->h);
+    free(handler);
+    return SPIKE_SUCCESS;
+}
+
+Error_t directSolver_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t *colind, integer_t *rowptr,
+                           double *aij, double *xij, double *bij)
+{
+    DirectSolverHander_t *h = directSolver_CreateHandler();
+    if (!h) return SPIKE_ERR_DEVICE;
+    int e = directSolver_Configure(h, nrhs);
+    if (!e) e = directSolver_Factorize(h, n, nnz, colind, rowptr, aij);
+    if (!e) e = directSolver_SolveForRHS(h, nrhs, xij, bij);
+    directSolver_Finalize(h);
+    return e;
+}
+
+Error_t directSolver_Host_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t *colind, integer_t *rowptr,
+                                double *aij, double *xij, double *bij)
+{
+    return directSolver_Solve(n, nnz, nrhs, colind, rowptr, aij, xij, bij);
+}
+
+Error_t ComputeResidualOfLinearSystem(integer_t *colind, integer_t *rowptr, double *aij, double *x, double *b,
+                                      integer_t n, integer_t nrhs)
+{
+    if (n <= 0 || nrhs < 0) return SPIKE_ERR_INVALID_ARG;
+    double *rel = (double *)malloc(sizeof(double) * (nrhs > 0 ? nrhs : 1));
+    if (!rel) return SPIKE_ERR_OOM;
+    int e = spike_residual_host(SPIKE_DTYPE_D, n, rowptr[n], rowptr, colind, aij, nrhs, x, b, rel);
+    for (integer_t c = 0; !e && c < nrhs && c < 5; ++c)
+        fprintf(stderr, "\n\tProcessing residual for %d-th RHS vector\n\t\tRelative residual %E\n", c + 1, rel[c]);
+    free(rel);
+    return e;
+}

@@ -1,0 +1,1121 @@
+// spike_core.hip -- host orchestration of the MI355X SPIKE solver and the spike_ext.h C ABI.
+//
+// Pipeline per batch of systems (a single-system handle is a batch of one):
+//   validate + bandwidth (one launch over every system's rows)
+//   group systems by block size b = 16 * NT, NT in {1, 2, 4, 8}
+//   per group: CSR -> blocks, block LU (k_factor), top block rows (k_rtop), V/W spike tips
+//              (k_sweep), reduced-system setup (k_red_setup)                    -- spike_factor
+//   per RHS chunk: y tips (k_sweep), block-Jacobi reduced solve in FP64 (k_red_jacobi),
+//              recovery into the FP64 solution (k_recover), FP64 residual (k_residual);
+//              mixed precision repeats on the residual until every column meets tol.  -- solve
+// The reference's flow is OpenMP/spike_interfaces.c:43-419; partition rows follow
+// OpenMP/spike_analysis.c:38-76.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <memory>
+
+#include "spike_dev.h"
+#include "spike_kernels.inc"
+#include "spike_kernels_aux.inc"
+#include "spike_ext.h"
+
+// ============================================================================ utilities
+namespace {
+
+thread_local char g_errmsg[512];
+
+void set_err(const char *fmt, const char *a = "", long v = 0) {
+    snprintf(g_errmsg, sizeof g_errmsg, fmt, a, v);
+}
+
+#define HIPCK(call)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            set_err("HIP error %s (line %ld)", hipGetErrorString(e_), (long)__LINE__); \
+            return SPIKE_ERR_DEVICE;                                                 \
+        }                                                                            \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t need) {
+        if (need == 0) need = 256;
+        if (bytes >= need) return 0;
+        release();
+        if (hipMalloc(&p, need) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            set_err("device allocation of %s%ld bytes failed", "", (long)need);
+            return SPIKE_ERR_OOM;
+        }
+        bytes = need;
+        return 0;
+    }
+    template <typename X> X *as() const { return (X *)p; }
+};
+
+// Host -> device copy of a small descriptor array; completes before returning because the host
+// vectors are transient or rewritten right after the call.
+int upload(DevBuf &d, const void *h, size_t bytes, hipStream_t s) {
+    int e = d.ensure(bytes);
+    if (e) return e;
+    if (bytes) {
+        HIPCK(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+        HIPCK(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+template <typename T> T s_zero_host() {
+    T z;
+    memset(&z, 0, sizeof z);
+    return z;
+}
+
+int block_nt(int ku, int kl) {
+    const int k = std::max(ku, kl);
+    if (k <= 16) return 1;
+    if (k <= 32) return 2;
+    if (k <= 64) return 4;
+    if (k <= 128) return 8;
+    return 0;
+}
+
+// tcode: 0 float, 1 double, 2 complex float, 3 complex double
+int type_code(int dtype, int precision) { return (dtype == SPIKE_DTYPE_Z ? 2 : 0) + (precision == SPIKE_PREC_FP64 ? 1 : 0); }
+size_t type_bytes(int tcode) { return tcode == 0 ? 4 : (tcode == 3 ? 16 : 8); }
+size_t h_bytes(int dtype) { return dtype == SPIKE_DTYPE_Z ? 16 : 8; }
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ============================================================================ autotuner
+// Partition count from (n, k, nrhs, dtype, precision) only, so a system's schedule never
+// depends on the batch it is solved in.  Each partition is one workgroup chain of nb blocks;
+// about 24 blocks per partition keep the spike couplings small enough for the reduced
+// block-Jacobi iteration to converge in a few sweeps while giving a 10^6-row, k=64 system
+// ~650 independent chains (2-3 per CU on 256 CUs).
+extern "C" int spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int precision) {
+    (void)nrhs; (void)dtype; (void)precision;
+    const int NT = block_nt(ku, kl);
+    if (NT == 0 || n <= 0) return 1;
+    const int b = 16 * NT;
+    const long rows_target = 24L * b;
+    long P = n / rows_target;
+    const long min_rows = std::max(2 * b, ku + kl);
+    P = std::min(P, (long)n / std::max(1L, min_rows));
+    P = std::max(1L, std::min(P, 1024L));
+    return (int)P;
+}
+
+// ============================================================================ engine
+namespace {
+
+struct SysHost {
+    int n = 0;
+    int64_t nnz = 0;
+    const int *d_rowptr = nullptr, *d_colind = nullptr;
+    const void *d_aij = nullptr;
+    int ku = 0, kl = 0, P = 1, NT = 1;
+    int group = -1, gidx = 0;     // group and index inside the group
+    int nrhs = 0;
+    const void *d_b = nullptr;
+    int ldb = 0;
+    void *d_x = nullptr;
+    int ldx = 0;
+    double maxrel = 0.0;
+    int refine_iters = 0, red_iters = 0;
+    double red_update = 0.0;
+};
+
+struct Group {
+    int NT = 1, b = 16, tcode = 1;
+    std::vector<int> sys;                   // batch indices
+    std::vector<PartDesc> parts;
+    std::vector<IfaceDesc> ifs;
+    std::vector<int> part_iface;
+    std::vector<RowChunk> chunks;
+    std::vector<int> chunk0;                // per system in the group: first chunk, +1 end
+    std::vector<SysDev> sysdev;
+    std::vector<int> sys_part0;             // global partition of each system
+    int64_t nblk = 0;
+    int64_t rows_pad = 0;                   // nblk * b
+    DevBuf d_parts, d_ifs, d_part_iface, d_chunks, d_chunk0, d_sysdev, d_active, d_red_active;
+    DevBuf D, U, C, R, Bt, Ct;              // factor type
+    DevBuf vt, vb, wt, wb, T;               // H type
+    DevBuf yt, yb, red0, red1, scratch, rwork, partials, norms, conv, pivinfo, status;
+    DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
+    int n_items_vw = 0, n_items_y = 0, n_items_r = 0;
+    int waves_vw = 4, waves_y = 1;
+    int ymax = 0;
+    int red_last = 0;                       // buffer (0: red0, 1: red1) holding the last iterate
+    std::vector<double> piv_host;
+};
+
+struct Engine {
+    spike_opts_t o;
+    int dtype = SPIKE_DTYPE_D;
+    hipStream_t stream = nullptr;
+    std::vector<SysHost> sys;
+    std::vector<std::unique_ptr<Group>> groups;
+    int ngroups = 0;
+    bool factored = false;
+    DevBuf d_bw, d_bwchunks, d_bwsys;
+    DevBuf own_rowptr, own_colind, own_aij;   // host-matrix copies (spike_set_matrix_host)
+    DevBuf stage_b, stage_x;                   // host solve staging
+    spike_stats_t st;
+    hipEvent_t ev[8];
+    bool events = false;
+    Engine() {
+        spike_default_opts(&o);
+        memset(&st, 0, sizeof st);
+    }
+    ~Engine() {
+        if (events)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    int init_events() {
+        if (events) return 0;
+        for (auto &e : ev) HIPCK(hipEventCreate(&e));
+        events = true;
+        return 0;
+    }
+    float ev_ms(int a, int b) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) { (void)hipGetLastError(); return 0.f; }
+        return ms;
+    }
+    size_t device_bytes() const {
+        size_t s = 0;
+        for (int gi = 0; gi < ngroups; ++gi) {
+            const Group &g = *groups[gi];
+            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
+                                  &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork};
+            for (auto b : bs) s += b->bytes;
+        }
+        return s;
+    }
+    int factor();
+    int solve_chunk(int c0, int m);
+    int solve(int nrhs_hint);
+};
+
+// ---------------------------------------------------------------------------- dispatch
+#define NT_SWITCH(NTV, BODY)                                 \
+    switch (NTV) {                                           \
+    case 1: { constexpr int NT = 1; BODY; } break;           \
+    case 2: { constexpr int NT = 2; BODY; } break;           \
+    case 4: { constexpr int NT = 4; BODY; } break;           \
+    case 8: { constexpr int NT = 8; BODY; } break;           \
+    default: break;                                          \
+    }
+
+#define T_SWITCH(TC, BODY)                                   \
+    switch (TC) {                                            \
+    case 0: { typedef float T; BODY; } break;                \
+    case 1: { typedef double T; BODY; } break;               \
+    case 2: { typedef zf T; BODY; } break;                   \
+    case 3: { typedef zd T; BODY; } break;                   \
+    default: break;                                          \
+    }
+
+template <typename T>
+int fill_zero(DevBuf &d, size_t count, hipStream_t s) {
+    if (!count) return 0;
+    const int grid = (int)std::min<size_t>((count + 255) / 256, 65536);
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid), dim3(256), 0, s, d.as<T>(), (int64_t)count, s_zero_host<T>());
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================ factor
+namespace {
+
+int Engine::factor() {
+    if (sys.empty()) { set_err("no matrix bound%s", ""); return SPIKE_ERR_INVALID_ARG; }
+    int e = init_events();
+    if (e) return e;
+    const double t0 = now_ms();
+    hipStream_t s = stream;
+    const int nsys = (int)sys.size();
+    HIPCK(hipEventRecord(ev[0], s));
+
+    // ---- validation and bandwidth of every system (one launch)
+    {
+        std::vector<RowChunk> ch;
+        std::vector<SysDev> sd(nsys);
+        for (int i = 0; i < nsys; ++i) {
+            memset(&sd[i], 0, sizeof(SysDev));
+            sd[i].rowptr = sys[i].d_rowptr;
+            sd[i].colind = sys[i].d_colind;
+            sd[i].n = sys[i].n;
+            for (int r = 0; r < sys[i].n; r += 256) ch.push_back({i, r});
+        }
+        if ((e = upload(d_bwchunks, ch.data(), ch.size() * sizeof(RowChunk), s))) return e;
+        if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
+        if ((e = d_bw.ensure(3 * sizeof(int) * nsys))) return e;
+        HIPCK(hipMemsetAsync(d_bw.p, 0, 3 * sizeof(int) * nsys, s));
+        if (!ch.empty())
+            hipLaunchKernelGGL(k_bandwidth, dim3((unsigned)ch.size()), dim3(256), 0, s, d_bwchunks.as<RowChunk>(),
+                               d_bwsys.as<SysDev>(), d_bw.as<int>());
+        HIPCK(hipGetLastError());
+        std::vector<int> bw(3 * nsys);
+        HIPCK(hipMemcpyAsync(bw.data(), d_bw.p, bw.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        for (int i = 0; i < nsys; ++i) {
+            if (bw[3 * i + 2]) { set_err("malformed CSR in system %s%ld", "", (long)i); return SPIKE_ERR_MALFORMED_CSR; }
+            SysHost &S = sys[i];
+            S.ku = bw[3 * i];
+            S.kl = bw[3 * i + 1];
+            S.NT = block_nt(S.ku, S.kl);
+            if (S.NT == 0 || (dtype == SPIKE_DTYPE_Z && S.NT > 4)) {
+                set_err("bandwidth %ld exceeds this build's limit%s", "", (long)std::max(S.ku, S.kl));
+                return SPIKE_ERR_UNSUPPORTED;
+            }
+            S.P = o.partitions > 0 ? o.partitions
+                                   : spike_autotune_partitions(S.n, S.ku, S.kl, S.nrhs, dtype, o.precision);
+            if (S.P > S.n) S.P = std::max(1, S.n);
+            const int nreg = S.n / S.P;
+            if (S.P > 1 && (nreg < S.ku + S.kl || nreg < std::max(S.ku, S.kl) || nreg < 1)) {
+                set_err("partitions too small for the bandwidth (rows per partition %ld)%s", "", (long)nreg);
+                return SPIKE_ERR_INVALID_ARG;
+            }
+        }
+    }
+    HIPCK(hipEventRecord(ev[1], s));
+
+    // ---- group systems by block size
+    const int tcode = type_code(dtype, o.precision);
+    const size_t ts = type_bytes(tcode), hs = h_bytes(dtype);
+    int nts[4] = {1, 2, 4, 8};
+    ngroups = 0;
+    for (int q = 0; q < 4; ++q) {
+        std::vector<int> members;
+        for (int i = 0; i < nsys; ++i)
+            if (sys[i].NT == nts[q]) members.push_back(i);
+        if (members.empty()) continue;
+        if ((int)groups.size() <= ngroups) groups.emplace_back(new Group());
+        Group &g = *groups[ngroups];
+        g.NT = nts[q];
+        g.b = 16 * g.NT;
+        g.tcode = tcode;
+        g.sys = members;
+        g.parts.clear(); g.ifs.clear(); g.part_iface.clear(); g.chunks.clear(); g.chunk0.clear();
+        g.sysdev.clear(); g.sys_part0.clear();
+        int64_t blk = 0;
+        for (int gs = 0; gs < (int)members.size(); ++gs) {
+            SysHost &S = sys[members[gs]];
+            S.group = ngroups;
+            S.gidx = gs;
+            const int P = S.P, nreg = S.n / P;
+            const int part0 = (int)g.parts.size(), if0 = (int)g.ifs.size();
+            g.sys_part0.push_back(part0);
+            for (int p = 0; p < P; ++p) {
+                PartDesc d;
+                memset(&d, 0, sizeof d);
+                d.sys = gs; d.p = p; d.P = P;
+                d.r0 = p * nreg;
+                d.ni = (p == P - 1) ? S.n - d.r0 : nreg;
+                d.nb = ceil_div(d.ni, g.b);
+                d.ku = S.ku; d.kl = S.kl;
+                d.blk = blk;
+                blk += d.nb;
+                g.parts.push_back(d);
+                g.part_iface.push_back(if0 + p);
+            }
+            for (int i = 0; i + 1 < P; ++i) {
+                IfaceDesc f;
+                memset(&f, 0, sizeof f);
+                f.sys = gs; f.i = i; f.P = P; f.ku = S.ku; f.kl = S.kl; f.part = part0 + i;
+                g.ifs.push_back(f);
+            }
+            g.chunk0.push_back((int)g.chunks.size());
+            for (int r = 0; r < S.n; r += 256) g.chunks.push_back({gs, r});
+            SysDev sd;
+            memset(&sd, 0, sizeof sd);
+            sd.rowptr = S.d_rowptr; sd.colind = S.d_colind; sd.aij = S.d_aij;
+            sd.n = S.n; sd.ku = S.ku; sd.kl = S.kl; sd.P = P; sd.part0 = part0; sd.nreg = nreg;
+            g.sysdev.push_back(sd);
+        }
+        g.chunk0.push_back((int)g.chunks.size());
+        g.nblk = blk;
+        g.rows_pad = blk * g.b;
+        ++ngroups;
+    }
+
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const size_t bb = (size_t)g.b * g.b;
+        const size_t np = g.parts.size(), nif = g.ifs.size();
+        if ((e = upload(g.d_parts, g.parts.data(), np * sizeof(PartDesc), s))) return e;
+        if ((e = upload(g.d_ifs, g.ifs.data(), nif * sizeof(IfaceDesc), s))) return e;
+        if ((e = upload(g.d_part_iface, g.part_iface.data(), g.part_iface.size() * sizeof(int), s))) return e;
+        if ((e = upload(g.d_chunks, g.chunks.data(), g.chunks.size() * sizeof(RowChunk), s))) return e;
+        if ((e = upload(g.d_chunk0, g.chunk0.data(), g.chunk0.size() * sizeof(int), s))) return e;
+        if ((e = upload(g.d_sysdev, g.sysdev.data(), g.sysdev.size() * sizeof(SysDev), s))) return e;
+        DevBuf *tb[] = {&g.D, &g.U, &g.C, &g.R};
+        for (auto b : tb)
+            if ((e = b->ensure(g.nblk * bb * ts))) return e;
+        if ((e = g.Bt.ensure(np * bb * ts)) || (e = g.Ct.ensure(np * bb * ts))) return e;
+        DevBuf *hb[] = {&g.vt, &g.vb, &g.wt, &g.wb};
+        for (auto b : hb)
+            if ((e = b->ensure(np * bb * hs))) return e;
+        if ((e = g.T.ensure(std::max<size_t>(nif, 1) * bb * hs))) return e;
+        if ((e = g.pivinfo.ensure(np * 2 * sizeof(double)))) return e;
+        if ((e = g.status.ensure(4 * sizeof(int)))) return e;
+        HIPCK(hipMemsetAsync(g.status.p, 0, 4 * sizeof(int), s));
+        // zero the scattered arrays (fill kernels; no memset nodes in code a caller may capture)
+        T_SWITCH(tcode, {
+            if ((e = fill_zero<T>(g.D, g.nblk * bb, s)) || (e = fill_zero<T>(g.U, g.nblk * bb, s)) ||
+                (e = fill_zero<T>(g.C, g.nblk * bb, s)) || (e = fill_zero<T>(g.Bt, np * bb, s)) ||
+                (e = fill_zero<T>(g.Ct, np * bb, s)))
+                return e;
+        });
+    }
+    HIPCK(hipEventRecord(ev[2], s));
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const int np = (int)g.parts.size();
+        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+            hipLaunchKernelGGL((k_csr_to_blocks<T, NT>), dim3((unsigned)g.chunks.size()), dim3(256), 0, s,
+                               g.d_chunks.as<RowChunk>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(),
+                               g.D.as<T>(), g.U.as<T>(), g.C.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(),
+                               g.status.as<int>(), (const int *)nullptr);
+        }));
+        HIPCK(hipGetLastError());
+        (void)np;
+    }
+    HIPCK(hipEventRecord(ev[3], s));
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const int np = (int)g.parts.size();
+        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+            hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
+                               g.U.as<T>(), (const T *)g.C.as<T>(), g.pivinfo.as<double>(), (const int *)nullptr);
+            hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
+                               (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr);
+        }));
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(ev[4], s));
+    // ---- V / W spike tips (partitions of systems with P > 1)
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        std::vector<SweepItem> items;
+        const int cpi = 64;
+        for (int q = 0; q < (int)g.parts.size(); ++q) {
+            const PartDesc &d = g.parts[q];
+            if (d.P == 1) continue;
+            SweepItem it;
+            memset(&it, 0, sizeof it);
+            it.part = q;
+            if (d.p < d.P - 1)
+                for (int c = 0; c < d.ku; c += cpi) { it.kind = ITEM_V; it.col0 = c; it.ncols = std::min(cpi, d.ku - c); items.push_back(it); }
+            if (d.p > 0)
+                for (int c = 0; c < d.kl; c += cpi) { it.kind = ITEM_W; it.col0 = c; it.ncols = std::min(cpi, d.kl - c); items.push_back(it); }
+        }
+        g.n_items_vw = (int)items.size();
+        if (items.empty()) continue;
+        if ((e = upload(g.items_vw, items.data(), items.size() * sizeof(SweepItem), s))) return e;
+        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+            typedef typename tr<T>::H H;
+            hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_vw), dim3(256), 0, s, g.items_vw.as<SweepItem>(),
+                               g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
+                               (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
+                               (const T *)g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
+                               (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+        }));
+        HIPCK(hipGetLastError());
+    }
+    // ---- reduced-system setup: T_i = (I - Wt_{i+1} Vb_i)^{-1}
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        if (g.ifs.empty()) continue;
+        const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
+        if (dtype == SPIKE_DTYPE_Z) {
+            HIPCK(hipFuncSetAttribute((const void *)k_red_setup<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            hipLaunchKernelGGL(k_red_setup<zd>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s, g.d_ifs.as<IfaceDesc>(),
+                               (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(), g.b, g.status.as<int>() + 1,
+                               (const int *)nullptr);
+        } else {
+            HIPCK(hipFuncSetAttribute((const void *)k_red_setup<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            hipLaunchKernelGGL(k_red_setup<double>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s,
+                               g.d_ifs.as<IfaceDesc>(), (const double *)g.vb.p, (const double *)g.wt.p, g.T.as<double>(),
+                               g.b, g.status.as<int>() + 1, (const int *)nullptr);
+        }
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(ev[5], s));
+    // ---- status: malformed entries, singular pivots
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        int stv[4];
+        g.piv_host.resize(g.parts.size() * 2);
+        HIPCK(hipMemcpyAsync(stv, g.status.p, sizeof stv, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(g.piv_host.data(), g.pivinfo.p, g.piv_host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (stv[0]) { set_err("matrix entry outside the detected band%s", ""); return SPIKE_ERR_MALFORMED_CSR; }
+        for (size_t q = 0; q < g.parts.size(); ++q)
+            if (g.piv_host[2 * q + 1] != 0.0) {
+                set_err("zero or non-finite pivot in partition %s%ld", "", (long)q);
+                return SPIKE_ERR_SINGULAR;
+            }
+        if (stv[1]) { set_err("singular reduced-system block%s", ""); return SPIKE_ERR_SINGULAR; }
+    }
+    factored = true;
+    st.ms_convert = ev_ms(2, 3);
+    st.ms_factor = ev_ms(3, 4);
+    st.ms_spikes = ev_ms(4, 5);
+    st.ms_factor_total = now_ms() - t0;
+    return SPIKE_SUCCESS;
+}
+
+// ============================================================================ solve
+int Engine::solve_chunk(int c0, int ymax) {
+    hipStream_t s = stream;
+    int e;
+    const size_t hs = h_bytes(dtype);
+    const double rtol = (o.precision == SPIKE_PREC_FP64) ? o.reduced_tol : std::max(o.reduced_tol, 1e-9);
+    const int max_refine = std::max(0, o.max_refine);
+    double t_y = 0, t_red = 0, t_rec = 0, t_res = 0;
+    // ---- per-group workspaces, descriptors and items
+    std::vector<std::vector<int>> active(ngroups);
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const size_t np = g.parts.size(), nif = g.ifs.size(), ngs = g.sys.size();
+        const size_t ts = type_bytes(g.tcode);
+        g.ymax = ymax;
+        if ((e = g.yt.ensure(np * g.b * ymax * hs)) || (e = g.yb.ensure(np * g.b * ymax * hs)) ||
+            (e = g.red0.ensure(std::max<size_t>(nif, 1) * 2 * g.b * ymax * hs)) ||
+            (e = g.red1.ensure(std::max<size_t>(nif, 1) * 2 * g.b * ymax * hs)) ||
+            (e = g.scratch.ensure((size_t)g.rows_pad * ymax * ts)) ||
+            (e = g.partials.ensure(g.chunks.size() * ymax * 2 * sizeof(double))) ||
+            (e = g.norms.ensure(ngs * ymax * 2 * sizeof(double))) ||
+            (e = g.conv.ensure(std::max<size_t>(nif, 1) * ceil_div(ymax, 16) * 2 * sizeof(double))))
+            return e;
+        size_t rrows = 0;
+        for (size_t gs = 0; gs < ngs; ++gs) rrows += sys[g.sys[gs]].n;
+        if ((e = g.rwork.ensure(rrows * ymax * hs))) return e;
+        active[gi].assign(ngs, 0);
+        size_t roff = 0;
+        for (size_t gs = 0; gs < ngs; ++gs) {
+            SysHost &S = sys[g.sys[gs]];
+            SysDev &sd = g.sysdev[gs];
+            const int m = std::max(0, std::min(ymax, S.nrhs - c0));
+            sd.m = m;
+            sd.ldb = S.ldb; sd.ldx = S.ldx; sd.ldr = S.n;
+            sd.bptr = (const char *)S.d_b + (size_t)c0 * S.ldb * hs;
+            sd.xptr = (char *)S.d_x + (size_t)c0 * S.ldx * hs;
+            sd.rptr = (char *)g.rwork.p + roff * ymax * hs;
+            roff += S.n;
+            active[gi][gs] = m > 0;
+            S.refine_iters = 0;
+        }
+        if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
+        // items: Y sweeps and recoveries, from b (first pass) and from r (refinement passes)
+        g.waves_y = ymax <= 16 ? 1 : (ymax <= 32 ? 2 : 4);
+        const int cpi = 16 * g.waves_y;
+        std::vector<SweepItem> yb_, yr_, rb_, rr_;
+        for (int q = 0; q < (int)np; ++q) {
+            const PartDesc &d = g.parts[q];
+            const SysDev &sd = g.sysdev[d.sys];
+            const SysHost &S = sys[g.sys[d.sys]];
+            for (int c = 0; c < sd.m; c += cpi) {
+                SweepItem it;
+                memset(&it, 0, sizeof it);
+                it.part = q; it.kind = ITEM_Y; it.col0 = c; it.ncols = std::min(cpi, sd.m - c);
+                it.dst = (char *)sd.xptr + (size_t)c * S.ldx * hs; it.lddst = S.ldx;
+                it.src = (const char *)sd.bptr + (size_t)c * S.ldb * hs; it.ldsrc = S.ldb;
+                if (d.P > 1) yb_.push_back(it);
+                rb_.push_back(it);
+                it.src = (const char *)sd.rptr + (size_t)c * S.n * hs; it.ldsrc = S.n;
+                if (d.P > 1) yr_.push_back(it);
+                rr_.push_back(it);
+            }
+        }
+        g.n_items_y = (int)yb_.size();
+        g.n_items_r = (int)rb_.size();
+        if ((e = upload(g.items_yb, yb_.data(), yb_.size() * sizeof(SweepItem), s)) ||
+            (e = upload(g.items_yr, yr_.data(), yr_.size() * sizeof(SweepItem), s)) ||
+            (e = upload(g.items_rb, rb_.data(), rb_.size() * sizeof(SweepItem), s)) ||
+            (e = upload(g.items_rr, rr_.data(), rr_.size() * sizeof(SweepItem), s)))
+            return e;
+        if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+        if ((e = g.d_red_active.ensure(ngs * sizeof(int)))) return e;
+    }
+
+    bool any = true;
+    for (int it = 0; any && it <= max_refine; ++it) {
+        double ta = now_ms();
+        HIPCK(hipEventRecord(ev[0], s));
+        // ---- y tips
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
+            if (!g.n_items_y) continue;
+            const DevBuf &items = it == 0 ? g.items_yb : g.items_yr;
+            T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+                typedef typename tr<T>::H H;
+                hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_y), dim3(64 * g.waves_y), 0, s, items.as<SweepItem>(),
+                                   g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
+                                   (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
+                                   (const T *)g.Ct.as<T>(), (H *)nullptr, (H *)nullptr, (H *)nullptr, (H *)nullptr,
+                                   g.yt.as<H>(), g.yb.as<H>(), g.ymax, g.d_active.as<int>());
+            }));
+            HIPCK(hipGetLastError());
+        }
+        HIPCK(hipEventRecord(ev[1], s));
+        // ---- reduced system: block-Jacobi sweeps in FP64 until each system's update is small
+        std::vector<DevBuf *> redcur(ngroups, nullptr);
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
+            if (g.ifs.empty()) { redcur[gi] = &g.red0; continue; }
+            const size_t nif = g.ifs.size(), ngs = g.sys.size();
+            const int ncc = ceil_div(ymax, 16);
+            const size_t slot_elems = (size_t)2 * g.b * ymax;
+            if (dtype == SPIKE_DTYPE_Z) { if ((e = fill_zero<zd>(g.red0, nif * slot_elems, s))) return e; }
+            else if ((e = fill_zero<double>(g.red0, nif * slot_elems, s))) return e;
+            std::vector<int> ract = active[gi];
+            if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
+            DevBuf *xo = &g.red0, *xn = &g.red1;
+            const size_t smem = (size_t)3 * g.b * 16 * hs;
+            std::vector<double> conv(nif * ncc * 2);
+            std::vector<double> dmax(ngs), vmax(ngs);
+            int iters = 0;
+            const int maxit = std::max(2, o.reduced_maxit);
+            while (iters < maxit) {
+                for (int rep = 0; rep < (iters == 0 ? 2 : 1); ++rep) {
+                    if (dtype == SPIKE_DTYPE_Z)
+                        hipLaunchKernelGGL(k_red_jacobi<zd>, dim3((unsigned)(nif * ncc)), dim3(256), smem, s,
+                                           g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p, (const zd *)g.vb.p,
+                                           (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p, (const zd *)g.yb.p,
+                                           (const zd *)g.T.p, (const zd *)xo->p, xn->as<zd>(), g.conv.as<double>(),
+                                           g.b, ymax, ymax, g.d_red_active.as<int>());
+                    else
+                        hipLaunchKernelGGL(k_red_jacobi<double>, dim3((unsigned)(nif * ncc)), dim3(256), smem, s,
+                                           g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p, (const double *)g.vb.p,
+                                           (const double *)g.wt.p, (const double *)g.wb.p, (const double *)g.yt.p,
+                                           (const double *)g.yb.p, (const double *)g.T.p, (const double *)xo->p,
+                                           xn->as<double>(), g.conv.as<double>(), g.b, ymax, ymax,
+                                           g.d_red_active.as<int>());
+                    HIPCK(hipGetLastError());
+                    std::swap(xo, xn);
+                    ++iters;
+                }
+                HIPCK(hipMemcpyAsync(conv.data(), g.conv.p, conv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+                HIPCK(hipStreamSynchronize(s));
+                std::fill(dmax.begin(), dmax.end(), 0.0);
+                std::fill(vmax.begin(), vmax.end(), 0.0);
+                for (size_t q = 0; q < nif * ncc; ++q) {
+                    const int gs = g.ifs[q / ncc].sys;
+                    dmax[gs] = std::max(dmax[gs], conv[2 * q]);
+                    vmax[gs] = std::max(vmax[gs], conv[2 * q + 1]);
+                }
+                bool more = false;
+                for (size_t gs = 0; gs < ngs; ++gs) {
+                    if (!ract[gs]) continue;
+                    SysHost &S = sys[g.sys[gs]];
+                    const double rel = vmax[gs] > 0 ? dmax[gs] / vmax[gs] : dmax[gs];
+                    S.red_iters = iters;
+                    S.red_update = rel;
+                    if (!(rel <= rtol)) more = true;
+                    else ract[gs] = 0;
+                    if (!std::isfinite(rel) || rel > 1e3) { ract[gs] = 0; S.red_update = rel; }
+                }
+                if (!more) break;
+                if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
+            }
+            redcur[gi] = xo;     // last written iterate
+            g.red_last = (xo == &g.red1) ? 1 : 0;
+        }
+        HIPCK(hipEventRecord(ev[2], s));
+        // ---- recovery into the FP64 solution
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
+            if (!g.n_items_r) continue;
+            const DevBuf &items = it == 0 ? g.items_rb : g.items_rr;
+            T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+                typedef typename tr<T>::H H;
+                hipLaunchKernelGGL((k_recover<T, NT>), dim3(g.n_items_r), dim3(64 * g.waves_y), 0, s, items.as<SweepItem>(),
+                                   g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
+                                   (const T *)g.U.as<T>(), (const T *)g.Bt.as<T>(), (const T *)g.Ct.as<T>(),
+                                   (const H *)redcur[gi]->p, g.d_part_iface.as<int>(), g.ymax, g.scratch.as<T>(),
+                                   (int64_t)g.rows_pad, it > 0 ? 1 : 0, g.d_active.as<int>());
+            }));
+            HIPCK(hipGetLastError());
+        }
+        HIPCK(hipEventRecord(ev[3], s));
+        // ---- FP64 residual of every column
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
+            if (dtype == SPIKE_DTYPE_Z)
+                hipLaunchKernelGGL(k_residual<zd>, dim3((unsigned)g.chunks.size()), dim3(256), 0, s, g.d_chunks.as<RowChunk>(),
+                                   g.d_sysdev.as<SysDev>(), g.partials.as<double>(), ymax, g.d_active.as<int>());
+            else
+                hipLaunchKernelGGL(k_residual<double>, dim3((unsigned)g.chunks.size()), dim3(256), 0, s,
+                                   g.d_chunks.as<RowChunk>(), g.d_sysdev.as<SysDev>(), g.partials.as<double>(), ymax,
+                                   g.d_active.as<int>());
+            hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)g.sys.size()), dim3(64), 0, s, g.d_chunk0.as<int>(),
+                               g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.norms.as<double>(), ymax);
+            HIPCK(hipGetLastError());
+        }
+        HIPCK(hipEventRecord(ev[4], s));
+        any = false;
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
+            const size_t ngs = g.sys.size();
+            std::vector<double> nr(ngs * ymax * 2);
+            HIPCK(hipMemcpyAsync(nr.data(), g.norms.p, nr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCK(hipStreamSynchronize(s));
+            bool changed = false;
+            for (size_t gs = 0; gs < ngs; ++gs) {
+                if (!active[gi][gs]) continue;
+                SysHost &S = sys[g.sys[gs]];
+                double mr = 0.0;
+                for (int c = 0; c < g.sysdev[gs].m; ++c) {
+                    const double rr = nr[(gs * ymax + c) * 2], bn = nr[(gs * ymax + c) * 2 + 1];
+                    const double rel = bn > 0 ? sqrt(rr / bn) : sqrt(rr);
+                    mr = std::isfinite(rel) ? std::max(mr, rel) : INFINITY;
+                }
+                S.refine_iters = it;
+                if (c0 == 0) S.maxrel = mr; else S.maxrel = std::max(S.maxrel, mr);
+                if (mr <= o.tol || it == max_refine || !std::isfinite(mr)) { active[gi][gs] = 0; changed = true; }
+                else any = true;
+            }
+            if (changed && any)
+                if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+        }
+        t_y += ev_ms(0, 1); t_red += ev_ms(1, 2); t_rec += ev_ms(2, 3); t_res += ev_ms(3, 4);
+        (void)ta;
+    }
+    st.ms_spikes = t_y;
+    st.ms_reduced = t_red;
+    st.ms_recover = t_rec;
+    st.ms_residual = t_res;
+    return SPIKE_SUCCESS;
+}
+
+int Engine::solve(int) {
+    if (!factored) {
+        int e = factor();
+        if (e) return e;
+    }
+    const double t0 = now_ms();
+    int maxn = 0;
+    for (auto &S : sys) { maxn = std::max(maxn, S.nrhs); S.maxrel = 0.0; }
+    const int chunk = 64;
+    for (int c0 = 0; c0 < maxn; c0 += chunk) {
+        const int e = solve_chunk(c0, std::min(chunk, maxn - c0));
+        if (e) return e;
+    }
+    st.ms_solve_total = now_ms() - t0;
+    for (auto &S : sys)
+        if (!(S.maxrel <= o.tol)) {
+            set_err("relative residual %s%ld e-12 above tolerance", "", (long)(S.maxrel * 1e12));
+            return SPIKE_ERR_NOT_CONVERGED;
+        }
+    return SPIKE_SUCCESS;
+}
+
+void fill_stats(const Engine &E, spike_stats_t *s) {
+    *s = E.st;
+    if (!E.sys.empty()) {
+        const SysHost &S = E.sys[0];
+        s->n = S.n; s->nnz = S.nnz; s->ku = S.ku; s->kl = S.kl; s->b = 16 * S.NT; s->P = S.P; s->nrhs = S.nrhs;
+        s->refine_iters = S.refine_iters; s->reduced_iters = S.red_iters; s->reduced_update = S.red_update;
+        double mr = 0;
+        for (auto &X : E.sys) mr = std::max(mr, X.maxrel);
+        s->max_rel_residual = mr;
+    }
+    s->precision = E.o.precision;
+    s->dtype = E.dtype;
+    s->device_bytes = E.device_bytes();
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+struct spike_handle {
+    Engine E;
+    DevBuf xr_host_dummy;
+};
+
+extern "C" {
+
+void spike_default_opts(spike_opts_t *o) {
+    memset(o, 0, sizeof *o);
+    o->precision = SPIKE_PREC_FP64;
+    o->partitions = 0;
+    o->max_refine = 8;
+    o->tol = 1e-10;
+    o->reduced_maxit = 64;
+    o->reduced_tol = 1e-15;
+    o->sweep_waves = 0;
+    o->verbose = 0;
+}
+
+const char *spike_error_string(int code) {
+    switch (code) {
+    case SPIKE_SUCCESS: return "success";
+    case SPIKE_ERR_INVALID_ARG: return "invalid argument";
+    case SPIKE_ERR_MALFORMED_CSR: return "malformed CSR";
+    case SPIKE_ERR_OOM: return "out of memory";
+    case SPIKE_ERR_SINGULAR: return "singular pivot";
+    case SPIKE_ERR_NOT_CONVERGED: return "not converged";
+    case SPIKE_ERR_UNSUPPORTED: return "unsupported";
+    case SPIKE_ERR_DEVICE: return "device error";
+    case SPIKE_ERR_IO: return "io error";
+    case SPIKE_ERR_CANCELLED: return "cancelled";
+    default: return "unknown";
+    }
+}
+
+const char *spike_last_error_message(void) { return g_errmsg; }
+
+int spike_create(spike_handle_t **h, const spike_opts_t *o) {
+    if (!h) return SPIKE_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        set_err("no HIP device%s", "");
+        *h = nullptr;
+        return SPIKE_ERR_DEVICE;
+    }
+    spike_handle *x = new (std::nothrow) spike_handle();
+    if (!x) return SPIKE_ERR_OOM;
+    if (o) x->E.o = *o;
+    *h = x;
+    return SPIKE_SUCCESS;
+}
+
+int spike_destroy(spike_handle_t *h) {
+    if (!h) return SPIKE_ERR_INVALID_ARG;
+    (void)hipStreamSynchronize(h->E.stream);
+    delete h;
+    return SPIKE_SUCCESS;
+}
+
+int spike_set_opts(spike_handle_t *h, const spike_opts_t *o) {
+    if (!h || !o) return SPIKE_ERR_INVALID_ARG;
+    const bool refactor = o->precision != h->E.o.precision || o->partitions != h->E.o.partitions;
+    h->E.o = *o;
+    if (refactor) h->E.factored = false;
+    return SPIKE_SUCCESS;
+}
+
+int spike_set_stream(spike_handle_t *h, void *stream) {
+    if (!h) return SPIKE_ERR_INVALID_ARG;
+    h->E.stream = (hipStream_t)stream;
+    return SPIKE_SUCCESS;
+}
+
+int spike_set_matrix_dev(spike_handle_t *h, int dtype, int n, int64_t nnz, const int *d_rowptr, const int *d_colind,
+                         const void *d_aij) {
+    if (!h || n <= 0 || nnz < 0 || nnz > INT32_MAX || !d_rowptr || (nnz > 0 && (!d_colind || !d_aij)) ||
+        (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
+        return SPIKE_ERR_INVALID_ARG;
+    Engine &E = h->E;
+    E.dtype = dtype;
+    E.sys.assign(1, SysHost());
+    SysHost &S = E.sys[0];
+    S.n = n; S.nnz = nnz; S.d_rowptr = d_rowptr; S.d_colind = d_colind; S.d_aij = d_aij;
+    E.factored = false;
+    return SPIKE_SUCCESS;
+}
+
+int spike_set_matrix_host(spike_handle_t *h, int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,
+                          const void *aij) {
+    if (!h || n <= 0 || nnz < 0 || nnz > INT32_MAX || !rowptr || (nnz > 0 && (!colind || !aij)) ||
+        (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
+        return SPIKE_ERR_INVALID_ARG;
+    if (matrix_CheckCSR(n, (uLong_t)nnz, colind, rowptr) != SPIKE_SUCCESS) return SPIKE_ERR_MALFORMED_CSR;
+    Engine &E = h->E;
+    const size_t hs = h_bytes(dtype);
+    int e;
+    if ((e = E.own_rowptr.ensure((size_t)(n + 1) * 4)) || (e = E.own_colind.ensure((size_t)nnz * 4)) ||
+        (e = E.own_aij.ensure((size_t)nnz * hs)))
+        return e;
+    HIPCK(hipMemcpy(E.own_rowptr.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCK(hipMemcpy(E.own_colind.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(E.own_aij.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
+    }
+    return spike_set_matrix_dev(h, dtype, n, nnz, E.own_rowptr.as<int>(), E.own_colind.as<int>(), E.own_aij.p);
+}
+
+int spike_factor(spike_handle_t *h) {
+    if (!h) return SPIKE_ERR_INVALID_ARG;
+    return h->E.factor();
+}
+
+int spike_solve_dev(spike_handle_t *h, int nrhs, const void *d_b, int ldb, void *d_x, int ldx) {
+    if (!h || h->E.sys.empty() || nrhs < 0 || (nrhs > 0 && (!d_b || !d_x))) return SPIKE_ERR_INVALID_ARG;
+    SysHost &S = h->E.sys[0];
+    if (ldb < S.n || ldx < S.n) return SPIKE_ERR_INVALID_ARG;
+    if (nrhs == 0) return SPIKE_SUCCESS;
+    S.nrhs = nrhs; S.d_b = d_b; S.ldb = ldb; S.d_x = d_x; S.ldx = ldx;
+    const int e = h->E.solve(nrhs);
+    if (h->E.o.verbose) {
+        spike_stats_t st;
+        fill_stats(h->E, &st);
+        fprintf(stderr, "[spike] n=%d ku=%d kl=%d b=%d P=%d nrhs=%d prec=%d refine=%d red_iters=%d maxrel=%.3e "
+                        "factor=%.3fms (conv %.3f lu %.3f spikes %.3f) solve=%.3fms\n",
+                st.n, st.ku, st.kl, st.b, st.P, st.nrhs, st.precision, st.refine_iters, st.reduced_iters,
+                st.max_rel_residual, st.ms_factor_total, st.ms_convert, st.ms_factor, st.ms_spikes, st.ms_solve_total);
+    }
+    return e;
+}
+
+int spike_solve_host(spike_handle_t *h, int nrhs, const void *b, int ldb, void *x, int ldx) {
+    if (!h || h->E.sys.empty() || nrhs < 0 || (nrhs > 0 && (!b || !x))) return SPIKE_ERR_INVALID_ARG;
+    Engine &E = h->E;
+    const int n = E.sys[0].n;
+    if (ldb < n || ldx < n) return SPIKE_ERR_INVALID_ARG;
+    if (nrhs == 0) return SPIKE_SUCCESS;
+    const size_t hs = h_bytes(E.dtype), bytes = (size_t)n * nrhs * hs;
+    int e;
+    if ((e = E.stage_b.ensure(bytes)) || (e = E.stage_x.ensure(bytes))) return e;
+    HIPCK(hipMemcpy2D(E.stage_b.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
+    const int rc = spike_solve_dev(h, nrhs, E.stage_b.p, n, E.stage_x.p, n);
+    if (rc == SPIKE_SUCCESS || rc == SPIKE_ERR_NOT_CONVERGED)
+        HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, E.stage_x.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+int spike_get_stats(const spike_handle_t *h, spike_stats_t *s) {
+    if (!h || !s) return SPIKE_ERR_INVALID_ARG;
+    fill_stats(h->E, s);
+    return SPIKE_SUCCESS;
+}
+
+int spike_get_schedule(const spike_handle_t *h, int *P, int *rows, int cap) {
+    if (!h || h->E.sys.empty() || !P) return SPIKE_ERR_INVALID_ARG;
+    const SysHost &S = h->E.sys[0];
+    *P = S.P;
+    if (rows) {
+        if (cap < S.P + 1) return SPIKE_ERR_INVALID_ARG;
+        const int nreg = S.n / S.P;
+        for (int p = 0; p < S.P; ++p) rows[p] = p * nreg;
+        rows[S.P] = S.n;
+    }
+    return SPIKE_SUCCESS;
+}
+
+// Compact per-partition tips (reference layout) from the b-strided device slots.
+int spike_get_tips(const spike_handle_t *h, double *Vt, double *Vb, double *Wt, double *Wb, double *yt, double *yb) {
+    if (!h || h->E.sys.empty() || !h->E.factored || h->E.dtype != SPIKE_DTYPE_D) return SPIKE_ERR_INVALID_ARG;
+    const Engine &E = h->E;
+    const SysHost &S = E.sys[0];
+    const Group &g = *E.groups[S.group];
+    const int b = g.b, P = S.P, ku = S.ku, kl = S.kl, m = std::min(S.nrhs, g.ymax);
+    const size_t bb = (size_t)b * b;
+    const int p0 = g.sys_part0[S.gidx];
+    auto fetch = [&](const DevBuf &d, size_t slot, int rows, int cols, double *out) -> int {
+        if (!out || rows == 0 || cols == 0) return 0;
+        std::vector<double> tmp(slot * P);
+        HIPCK(hipMemcpy(tmp.data(), (const double *)d.p + (size_t)p0 * slot, tmp.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+        for (int p = 0; p < P; ++p)
+            for (int c = 0; c < cols; ++c)
+                for (int r = 0; r < rows; ++r) out[(size_t)p * rows * cols + r + (size_t)c * rows] = tmp[p * slot + r + (size_t)c * b];
+        return 0;
+    };
+    int e;
+    if ((e = fetch(g.vt, bb, ku, ku, Vt)) || (e = fetch(g.vb, bb, kl, ku, Vb)) || (e = fetch(g.wt, bb, ku, kl, Wt)) ||
+        (e = fetch(g.wb, bb, kl, kl, Wb)))
+        return e;
+    if (g.ymax > 0) {
+        if ((e = fetch(g.yt, (size_t)b * g.ymax, ku, m, yt)) || (e = fetch(g.yb, (size_t)b * g.ymax, kl, m, yb))) return e;
+    }
+    // the reference leaves V of the last partition and W of the first unset: report zeros
+    if (Vt && P > 0) memset(Vt + (size_t)(P - 1) * ku * ku, 0, sizeof(double) * ku * ku);
+    if (Vb && P > 0) memset(Vb + (size_t)(P - 1) * kl * ku, 0, sizeof(double) * kl * ku);
+    if (Wt) memset(Wt, 0, sizeof(double) * ku * kl);
+    if (Wb) memset(Wb, 0, sizeof(double) * kl * kl);
+    return SPIKE_SUCCESS;
+}
+
+// Reference-order reduced solution [x^t_p ; x^b_p] of every partition for the last inner solve
+// (the solve of A x = b itself when no refinement step followed), from the last interface
+// iterate and the tips; the end unknowns x^t_0 and x^b_{P-1} are formed on the host.
+int spike_get_reduced_solution(const spike_handle_t *h, double *xr) {
+    if (!h || !xr || h->E.sys.empty() || !h->E.factored || h->E.dtype != SPIKE_DTYPE_D) return SPIKE_ERR_INVALID_ARG;
+    const Engine &E = h->E;
+    const SysHost &S = E.sys[0];
+    const Group &g = *E.groups[S.group];
+    const int P = S.P, ku = S.ku, kl = S.kl, k2 = ku + kl, m = std::min(S.nrhs, g.ymax), b = g.b;
+    if (P < 2 || m < 1) return SPIKE_ERR_INVALID_ARG;
+    std::vector<double> Vt((size_t)P * ku * ku), Vb((size_t)P * kl * ku), Wt((size_t)P * ku * kl), Wb((size_t)P * kl * kl),
+        yt((size_t)P * ku * m), yb((size_t)P * kl * m);
+    int e = spike_get_tips(h, Vt.data(), Vb.data(), Wt.data(), Wb.data(), yt.data(), yb.data());
+    if (e) return e;
+    const size_t slot = (size_t)2 * b * g.ymax;
+    std::vector<double> X((size_t)(P - 1) * slot);
+    const int f0 = g.part_iface[g.sys_part0[S.gidx]];
+    const DevBuf &last = g.red_last ? g.red1 : g.red0;
+    HIPCK(hipMemcpy(X.data(), (const double *)last.p + (size_t)f0 * slot, X.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < m; ++c) {
+        double *col = xr + (size_t)c * P * k2;
+        for (int p = 0; p < P; ++p) {
+            for (int r = 0; r < ku; ++r) {
+                double v;
+                if (p > 0) v = X[(size_t)(p - 1) * slot + b + r + (size_t)c * 2 * b];
+                else {
+                    v = yt[r + (size_t)c * ku];
+                    for (int k = 0; k < ku; ++k) v -= Vt[r + (size_t)k * ku] * X[b + k + (size_t)c * 2 * b];
+                }
+                col[p * k2 + r] = v;
+            }
+            for (int r = 0; r < kl; ++r) {
+                double v;
+                if (p < P - 1) v = X[(size_t)p * slot + r + (size_t)c * 2 * b];
+                else {
+                    v = yb[(size_t)p * kl * m + r + (size_t)c * kl];
+                    for (int k = 0; k < kl; ++k)
+                        v -= Wb[(size_t)p * kl * kl + r + (size_t)k * kl] * X[(size_t)(p - 1) * slot + k + (size_t)c * 2 * b];
+                }
+                col[p * k2 + ku + r] = v;
+            }
+        }
+    }
+    return SPIKE_SUCCESS;
+}
+
+}  // extern "C"
+
+// ============================================================================ batched systems
+struct spike_batch {
+    Engine E;
+};
+
+extern "C" {
+
+int spike_batch_create(spike_batch_t **bt, const spike_opts_t *o) {
+    if (!bt) return SPIKE_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); *bt = nullptr; return SPIKE_ERR_DEVICE; }
+    spike_batch *x = new (std::nothrow) spike_batch();
+    if (!x) return SPIKE_ERR_OOM;
+    if (o) x->E.o = *o;
+    *bt = x;
+    return SPIKE_SUCCESS;
+}
+
+int spike_batch_destroy(spike_batch_t *bt) {
+    if (!bt) return SPIKE_ERR_INVALID_ARG;
+    (void)hipStreamSynchronize(bt->E.stream);
+    delete bt;
+    return SPIKE_SUCCESS;
+}
+
+int spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_system_t *sys) {
+    if (!bt || nsys <= 0 || !sys || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z)) return SPIKE_ERR_INVALID_ARG;
+    Engine &E = bt->E;
+    E.dtype = dtype;
+    E.sys.assign(nsys, SysHost());
+    for (int i = 0; i < nsys; ++i) {
+        const spike_system_t &q = sys[i];
+        if (q.n <= 0 || q.nnz < 0 || q.nnz > INT32_MAX || !q.d_rowptr || q.nrhs < 0 || (q.nrhs > 0 && (!q.d_b || !q.d_x)))
+            return SPIKE_ERR_INVALID_ARG;
+        SysHost &S = E.sys[i];
+        S.n = q.n; S.nnz = q.nnz; S.d_rowptr = q.d_rowptr; S.d_colind = q.d_colind; S.d_aij = q.d_aij;
+        S.nrhs = q.nrhs; S.d_b = q.d_b; S.ldb = q.n; S.d_x = q.d_x; S.ldx = q.n;
+    }
+    E.factored = false;
+    const double t0 = now_ms();
+    int e = E.factor();
+    if (e) return e;
+    e = E.solve(0);
+    if (E.o.verbose) {
+        double mr = 0;
+        for (auto &S : E.sys) mr = std::max(mr, S.maxrel);
+        fprintf(stderr, "[spike batch] %d systems, %d groups, factor %.3f ms, solve %.3f ms, total %.3f ms, max rel %.3e\n",
+                nsys, E.ngroups, E.st.ms_factor_total, E.st.ms_solve_total, now_ms() - t0, mr);
+    }
+    return e;
+}
+
+int spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s, double *rel) {
+    if (!bt) return SPIKE_ERR_INVALID_ARG;
+    if (s) fill_stats(bt->E, s);
+    if (rel)
+        for (size_t i = 0; i < bt->E.sys.size(); ++i) rel[i] = bt->E.sys[i].maxrel;
+    return SPIKE_SUCCESS;
+}
+
+// ============================================================================ residual helper
+int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind, const void *aij, int nrhs,
+                        const void *x, const void *b, double *rel) {
+    if (n <= 0 || nnz < 0 || nrhs < 0 || !rowptr || !rel || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
+        return SPIKE_ERR_INVALID_ARG;
+    if (nrhs == 0) return SPIKE_SUCCESS;
+    const size_t hs = h_bytes(dtype);
+    DevBuf drp, dci, da, dx, db, dr, dch, dsys, dpart, dnorm, dc0;
+    int e;
+    if ((e = drp.ensure((size_t)(n + 1) * 4)) || (e = dci.ensure((size_t)nnz * 4)) || (e = da.ensure((size_t)nnz * hs)) ||
+        (e = dx.ensure((size_t)n * nrhs * hs)) || (e = db.ensure((size_t)n * nrhs * hs)) || (e = dr.ensure((size_t)n * nrhs * hs)))
+        return e;
+    HIPCK(hipMemcpy(drp.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCK(hipMemcpy(dci.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(da.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
+    }
+    HIPCK(hipMemcpy(dx.p, x, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(db.p, b, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
+    std::vector<RowChunk> ch;
+    for (int r = 0; r < n; r += 256) ch.push_back({0, r});
+    const int nch = (int)ch.size();
+    for (int c0 = 0; c0 < nrhs; c0 += 64) {
+        const int m = std::min(64, nrhs - c0);
+        SysDev sd;
+        memset(&sd, 0, sizeof sd);
+        sd.rowptr = drp.as<int>(); sd.colind = dci.as<int>(); sd.aij = da.p; sd.n = n; sd.m = m;
+        sd.ldb = n; sd.ldx = n; sd.ldr = n;
+        sd.bptr = (char *)db.p + (size_t)c0 * n * hs;
+        sd.xptr = (char *)dx.p + (size_t)c0 * n * hs;
+        sd.rptr = (char *)dr.p + (size_t)c0 * n * hs;
+        int c0s[2] = {0, nch};
+        if ((e = upload(dch, ch.data(), ch.size() * sizeof(RowChunk), 0)) || (e = upload(dsys, &sd, sizeof sd, 0)) ||
+            (e = upload(dc0, c0s, sizeof c0s, 0)) || (e = dpart.ensure((size_t)nch * 64 * 2 * 8)) || (e = dnorm.ensure(64 * 2 * 8)))
+            return e;
+        if (dtype == SPIKE_DTYPE_Z)
+            hipLaunchKernelGGL(k_residual<zd>, dim3(nch), dim3(256), 0, 0, dch.as<RowChunk>(), dsys.as<SysDev>(),
+                               dpart.as<double>(), 64, (const int *)nullptr);
+        else
+            hipLaunchKernelGGL(k_residual<double>, dim3(nch), dim3(256), 0, 0, dch.as<RowChunk>(), dsys.as<SysDev>(),
+                               dpart.as<double>(), 64, (const int *)nullptr);
+        hipLaunchKernelGGL(k_norm_reduce, dim3(1), dim3(64), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),
+                           dnorm.as<double>(), 64);
+        HIPCK(hipGetLastError());
+        double nr[128];
+        HIPCK(hipMemcpy(nr, dnorm.p, sizeof nr, hipMemcpyDeviceToHost));
+        for (int c = 0; c < m; ++c) rel[c0 + c] = nr[2 * c + 1] > 0 ? sqrt(nr[2 * c] / nr[2 * c + 1]) : sqrt(nr[2 * c]);
+    }
+    return SPIKE_SUCCESS;
+}
+
+}  // extern "C"

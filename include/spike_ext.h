@@ -106,8 +106,10 @@ int  spike_get_reduced_solution(const spike_handle_t *h, double *xr);
 /* Set the HIP stream (hipStream_t passed as void*) the handle launches on; NULL = default. */
 int  spike_set_stream(spike_handle_t *h, void *stream);
 
-/* Name of the last error of this handle (static string). */
+/* Static name of an error code. */
 const char *spike_error_string(int code);
+/* Detail message of the last error raised on the calling thread. */
+const char *spike_last_error_message(void);
 
 /* ---------------- batched solve of independent systems (one set of launches) ----------- */
 typedef struct {

@@ -46,7 +46,8 @@ class Stats(ctypes.Structure):
         ("precision", ctypes.c_int), ("dtype", ctypes.c_int), ("refine_iters", ctypes.c_int),
         ("reduced_iters", ctypes.c_int), ("reduced_update", ctypes.c_double),
         ("max_rel_residual", ctypes.c_double), ("ms_convert", ctypes.c_double),
-        ("ms_factor", ctypes.c_double), ("ms_spikes", ctypes.c_double), ("ms_reduced", ctypes.c_double),
+        ("ms_factor", ctypes.c_double), ("ms_rtop", ctypes.c_double), ("ms_spikes", ctypes.c_double),
+        ("ms_red_setup", ctypes.c_double), ("ms_ysweep", ctypes.c_double), ("ms_reduced", ctypes.c_double),
         ("ms_recover", ctypes.c_double), ("ms_residual", ctypes.c_double),
         ("ms_factor_total", ctypes.c_double), ("ms_solve_total", ctypes.c_double),
         ("device_bytes", ctypes.c_size_t),
@@ -69,7 +70,7 @@ EXPORTED = [
     # spike_interfaces.h
     "sspike_core_host", "dspike_core_host", "cspike_core_host", "zspike_core_host", "dspike_core_host_blocking",
     # spike_matrix.h
-    "matrix_LoadCSR", "matrix_SaveCSR", "matrix_LoadCSR_z", "matrix_Deallocate", "matrix_z_Deallocate",
+    "matrix_LoadCSR", "matrix_SaveCSR", "matrix_LoadCSR_z", "matrix_LoadCSR_zl", "matrix_Deallocate", "matrix_z_Deallocate",
     "matrix_CreateEmptyMatrix", "matrix_CreateFromComponents", "matrix_AreEqual", "matrix_CheckCSR",
     "matrix_ComputeBandwidth", "matrix_ExtractMatrix", "matrix_ExtractBlock", "block_CreateEmptyBlock",
     "block_Deallocate", "spike_solve_analysis_p", "schedule_Destroy", "matrix_CreateEmptyReducedSystem",
@@ -149,6 +150,7 @@ def load():
         "matrix_Deallocate": (i32, [vp]),
         "matrix_LoadCSR": (vp, [ctypes.c_char_p]),
         "matrix_LoadCSR_z": (vp, [ctypes.c_char_p]),
+        "matrix_LoadCSR_zl": (vp, [ctypes.c_char_p, i32]),
         "matrix_z_Deallocate": (i32, [vp]),
         "matrix_SaveCSR": (i32, [ctypes.c_char_p, vp]),
         "matrix_ExtractMatrix": (vp, [vp, i32, i32, i32, i32, i32]),

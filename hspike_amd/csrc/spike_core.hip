@@ -123,7 +123,12 @@ extern "C" int spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dt
     const int NT = block_nt(ku, kl);
     if (NT == 0 || n <= 0) return 1;
     const int b = 16 * NT;
-    const long rows_target = 24L * b;
+    static const long blocks_per_part = [] {
+        const char *v = getenv("HSPIKE_BLOCKS_PER_PART");
+        const long x = v ? atol(v) : 0;
+        return x > 0 ? x : 24L;
+    }();
+    const long rows_target = blocks_per_part * b;
     long P = n / rows_target;
     const long min_rows = std::max(2 * b, ku + kl);
     P = std::min(P, (long)n / std::max(1L, min_rows));
@@ -166,11 +171,12 @@ struct Group {
     DevBuf d_parts, d_ifs, d_part_iface, d_chunks, d_chunk0, d_sysdev, d_active, d_red_active;
     DevBuf D, U, C, R, Bt, Ct;              // factor type
     DevBuf vt, vb, wt, wb, T;               // H type
-    DevBuf yt, yb, red0, red1, scratch, rwork, partials, norms, conv, pivinfo, status;
+    DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
     DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
     int n_items_vw = 0, n_items_y = 0, n_items_r = 0;
     int waves_vw = 4, waves_y = 1;
     int ymax = 0;
+    std::vector<uint64_t> solve_key;        // descriptors of the last solve chunk (reused when equal)
     int red_last = 0;                       // buffer (0: red0, 1: red1) holding the last iterate
     std::vector<double> piv_host;
 };
@@ -187,7 +193,7 @@ struct Engine {
     DevBuf own_rowptr, own_colind, own_aij;   // host-matrix copies (spike_set_matrix_host)
     DevBuf stage_b, stage_x;                   // host solve staging
     spike_stats_t st;
-    hipEvent_t ev[8];
+    hipEvent_t ev[16];
     bool events = false;
     Engine() {
         spike_default_opts(&o);
@@ -274,7 +280,7 @@ int Engine::factor() {
             sd[i].rowptr = sys[i].d_rowptr;
             sd[i].colind = sys[i].d_colind;
             sd[i].n = sys[i].n;
-            for (int r = 0; r < sys[i].n; r += 256) ch.push_back({i, r});
+            for (int r = 0; r < sys[i].n; r += RES_ROWS) ch.push_back({i, r});
         }
         if ((e = upload(d_bwchunks, ch.data(), ch.size() * sizeof(RowChunk), s))) return e;
         if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
@@ -355,7 +361,7 @@ int Engine::factor() {
                 g.ifs.push_back(f);
             }
             g.chunk0.push_back((int)g.chunks.size());
-            for (int r = 0; r < S.n; r += 256) g.chunks.push_back({gs, r});
+            for (int r = 0; r < S.n; r += RES_ROWS) g.chunks.push_back({gs, r});
             SysDev sd;
             memset(&sd, 0, sizeof sd);
             sd.rowptr = S.d_rowptr; sd.colind = S.d_colind; sd.aij = S.d_aij;
@@ -363,6 +369,7 @@ int Engine::factor() {
             g.sysdev.push_back(sd);
         }
         g.chunk0.push_back((int)g.chunks.size());
+        g.solve_key.clear();
         g.nblk = blk;
         g.rows_pad = blk * g.b;
         ++ngroups;
@@ -417,12 +424,20 @@ int Engine::factor() {
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                g.U.as<T>(), (const T *)g.C.as<T>(), g.pivinfo.as<double>(), (const int *)nullptr);
+        }));
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(ev[4], s));
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const int np = (int)g.parts.size();
+        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
                                (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr);
         }));
         HIPCK(hipGetLastError());
     }
-    HIPCK(hipEventRecord(ev[4], s));
+    HIPCK(hipEventRecord(ev[5], s));
     // ---- V / W spike tips (partitions of systems with P > 1)
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
@@ -452,6 +467,7 @@ int Engine::factor() {
         }));
         HIPCK(hipGetLastError());
     }
+    HIPCK(hipEventRecord(ev[6], s));
     // ---- reduced-system setup: T_i = (I - Wt_{i+1} Vb_i)^{-1}
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
@@ -470,7 +486,7 @@ int Engine::factor() {
         }
         HIPCK(hipGetLastError());
     }
-    HIPCK(hipEventRecord(ev[5], s));
+    HIPCK(hipEventRecord(ev[7], s));
     // ---- status: malformed entries, singular pivots
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
@@ -490,7 +506,9 @@ int Engine::factor() {
     factored = true;
     st.ms_convert = ev_ms(2, 3);
     st.ms_factor = ev_ms(3, 4);
-    st.ms_spikes = ev_ms(4, 5);
+    st.ms_rtop = ev_ms(4, 5);
+    st.ms_spikes = ev_ms(5, 6);
+    st.ms_red_setup = ev_ms(6, 7);
     st.ms_factor_total = now_ms() - t0;
     return SPIKE_SUCCESS;
 }
@@ -520,8 +538,16 @@ int Engine::solve_chunk(int c0, int ymax) {
             return e;
         size_t rrows = 0;
         for (size_t gs = 0; gs < ngs; ++gs) rrows += sys[g.sys[gs]].n;
-        if ((e = g.rwork.ensure(rrows * ymax * hs))) return e;
+        if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(rrows * ymax * hs))) return e;
         active[gi].assign(ngs, 0);
+        std::vector<uint64_t> key = {(uint64_t)c0, (uint64_t)ymax, (uint64_t)(uintptr_t)g.rwork.p,
+                                     (uint64_t)(uintptr_t)g.xrm.p};
+        for (size_t gs = 0; gs < ngs; ++gs) {
+            const SysHost &S = sys[g.sys[gs]];
+            key.insert(key.end(), {(uint64_t)(uintptr_t)S.d_b, (uint64_t)S.ldb, (uint64_t)(uintptr_t)S.d_x,
+                                   (uint64_t)S.ldx, (uint64_t)S.nrhs});
+        }
+        const bool cached = key == g.solve_key;
         size_t roff = 0;
         for (size_t gs = 0; gs < ngs; ++gs) {
             SysHost &S = sys[g.sys[gs]];
@@ -532,9 +558,15 @@ int Engine::solve_chunk(int c0, int ymax) {
             sd.bptr = (const char *)S.d_b + (size_t)c0 * S.ldb * hs;
             sd.xptr = (char *)S.d_x + (size_t)c0 * S.ldx * hs;
             sd.rptr = (char *)g.rwork.p + roff * ymax * hs;
+            sd.xrm = (char *)g.xrm.p + roff * ymax * hs;
+            sd.ldrm = ymax;
             roff += S.n;
             active[gi][gs] = m > 0;
             S.refine_iters = 0;
+        }
+        if (cached) {
+            if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+            continue;
         }
         if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
         // items: Y sweeps and recoveries, from b (first pass) and from r (refinement passes)
@@ -550,6 +582,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                 memset(&it, 0, sizeof it);
                 it.part = q; it.kind = ITEM_Y; it.col0 = c; it.ncols = std::min(cpi, sd.m - c);
                 it.dst = (char *)sd.xptr + (size_t)c * S.ldx * hs; it.lddst = S.ldx;
+                it.dst_rm = (void *)sd.xrm; it.ldrm = ymax;
                 it.src = (const char *)sd.bptr + (size_t)c * S.ldb * hs; it.ldsrc = S.ldb;
                 if (d.P > 1) yb_.push_back(it);
                 rb_.push_back(it);
@@ -567,12 +600,13 @@ int Engine::solve_chunk(int c0, int ymax) {
             return e;
         if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
         if ((e = g.d_red_active.ensure(ngs * sizeof(int)))) return e;
+        g.solve_key = key;
     }
 
     bool any = true;
     for (int it = 0; any && it <= max_refine; ++it) {
         double ta = now_ms();
-        HIPCK(hipEventRecord(ev[0], s));
+        HIPCK(hipEventRecord(ev[8], s));
         // ---- y tips
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -588,7 +622,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             }));
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[1], s));
+        HIPCK(hipEventRecord(ev[9], s));
         // ---- reduced system: block-Jacobi sweeps in FP64 until each system's update is small
         std::vector<DevBuf *> redcur(ngroups, nullptr);
         for (int gi = 0; gi < ngroups; ++gi) {
@@ -602,7 +636,11 @@ int Engine::solve_chunk(int c0, int ymax) {
             std::vector<int> ract = active[gi];
             if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
             DevBuf *xo = &g.red0, *xn = &g.red1;
-            const size_t smem = (size_t)3 * g.b * 16 * hs;
+            const size_t smem = (size_t)5 * g.b * 16 * hs;
+            if (dtype == SPIKE_DTYPE_Z)
+                HIPCK(hipFuncSetAttribute((const void *)k_red_jacobi<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            else
+                HIPCK(hipFuncSetAttribute((const void *)k_red_jacobi<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
             std::vector<double> conv(nif * ncc * 2);
             std::vector<double> dmax(ngs), vmax(ngs);
             int iters = 0;
@@ -652,7 +690,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             redcur[gi] = xo;     // last written iterate
             g.red_last = (xo == &g.red1) ? 1 : 0;
         }
-        HIPCK(hipEventRecord(ev[2], s));
+        HIPCK(hipEventRecord(ev[10], s));
         // ---- recovery into the FP64 solution
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -668,7 +706,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             }));
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[3], s));
+        HIPCK(hipEventRecord(ev[11], s));
         // ---- FP64 residual of every column
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -679,11 +717,11 @@ int Engine::solve_chunk(int c0, int ymax) {
                 hipLaunchKernelGGL(k_residual<double>, dim3((unsigned)g.chunks.size()), dim3(256), 0, s,
                                    g.d_chunks.as<RowChunk>(), g.d_sysdev.as<SysDev>(), g.partials.as<double>(), ymax,
                                    g.d_active.as<int>());
-            hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)g.sys.size()), dim3(64), 0, s, g.d_chunk0.as<int>(),
+            hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(g.sys.size() * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
                                g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.norms.as<double>(), ymax);
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[4], s));
+        HIPCK(hipEventRecord(ev[12], s));
         any = false;
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -709,10 +747,9 @@ int Engine::solve_chunk(int c0, int ymax) {
             if (changed && any)
                 if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
         }
-        t_y += ev_ms(0, 1); t_red += ev_ms(1, 2); t_rec += ev_ms(2, 3); t_res += ev_ms(3, 4);
-        (void)ta;
+        t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 12);
     }
-    st.ms_spikes = t_y;
+    st.ms_ysweep = t_y;
     st.ms_reduced = t_red;
     st.ms_recover = t_rec;
     st.ms_residual = t_res;
@@ -1087,7 +1124,7 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
     HIPCK(hipMemcpy(dx.p, x, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(db.p, b, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
     std::vector<RowChunk> ch;
-    for (int r = 0; r < n; r += 256) ch.push_back({0, r});
+    for (int r = 0; r < n; r += RES_ROWS) ch.push_back({0, r});
     const int nch = (int)ch.size();
     for (int c0 = 0; c0 < nrhs; c0 += 64) {
         const int m = std::min(64, nrhs - c0);
@@ -1108,7 +1145,7 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
         else
             hipLaunchKernelGGL(k_residual<double>, dim3(nch), dim3(256), 0, 0, dch.as<RowChunk>(), dsys.as<SysDev>(),
                                dpart.as<double>(), 64, (const int *)nullptr);
-        hipLaunchKernelGGL(k_norm_reduce, dim3(1), dim3(64), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),
+        hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),
                            dnorm.as<double>(), 64);
         HIPCK(hipGetLastError());
         double nr[128];

@@ -158,6 +158,9 @@ struct SweepItem {
     void *dst;         // recovery: solution (H type) offset to column col0, row 0 of the system
     int lddst;
     int pad1_;
+    void *dst_rm;      // recovery: row-major copy of the solution (H type, row stride ldrm), row 0, column 0
+    int ldrm;
+    int pad2_;
 };
 
 // Per-system view used by the row-parallel kernels (validation, conversion, residual).
@@ -171,8 +174,11 @@ struct SysDev {
     int m;              // RHS columns of the current chunk
     int ldb, ldx, ldr;
     const void *bptr;   // RHS (H), offset to the chunk's first column
-    void *xptr;         // solution (H)
+    void *xptr;         // solution (H), column-major with leading dimension ldx
     void *rptr;         // residual work (H)
+    const void *xrm;    // row-major copy of the solution (row stride ldrm), or null to read xptr
+    int ldrm;
+    int pad_;
 };
 
 struct RowChunk {

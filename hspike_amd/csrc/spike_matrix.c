@@ -5,9 +5,10 @@
  *   int32 n, int32 nnz, int32 dtype (0 real, 1 complex), aij[nnz], int32 colind[nnz],
  *   int32 rowptr[n+1].
  * Complex files written by the reference's generator (generator_banded.py:134-141) store all
- * real parts, then all imaginary parts; matrix_LoadCSR_z accepts that planar layout and the
- * interleaved layout the reference's loader assumes, and picks the one whose decoding is
- * consistent (the file sizes are identical, so the choice is made on the values).
+ * real parts, then all imaginary parts.  The reference's loader reads interleaved complex_t
+ * (spike_matrix.c:34-35) and so misreads those files; matrix_LoadCSR_z reads the planar layout
+ * its writer produces, matrix_LoadCSR_zl takes the layout explicitly (both layouts have the same
+ * file size, so it cannot be inferred from the file).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -80,25 +81,7 @@ Error_t matrix_SaveCSR(const char *filename, const matrix_t *M)
     return ok ? SPIKE_SUCCESS : SPIKE_ERR_IO;
 }
 
-/* Score a decoding of the complex values: the generator writes re, im ~ U[0,1) + shifts, the
- * diagonal being the largest; a wrong decoding mixes the real and imaginary streams.  The
- * planar decoding is chosen when its diagonal entries dominate their rows at least as often
- * as under the interleaved decoding. */
-static long dominance_score(integer_t n, const integer_t *rowptr, const integer_t *colind, const complex16 *a)
-{
-    long score = 0;
-    for (integer_t r = 0; r < n; ++r) {
-        double d = 0.0, off = 0.0;
-        for (integer_t idx = rowptr[r]; idx < rowptr[r + 1]; ++idx) {
-            double m = hypot(a[idx].real, a[idx].imag);
-            if (colind[idx] == r) d = m; else off = m > off ? m : off;
-        }
-        score += d >= off;
-    }
-    return score;
-}
-
-matrix_z_t *matrix_LoadCSR_z(const char *filename)
+matrix_z_t *matrix_LoadCSR_zl(const char *filename, int planar)
 {
     FILE *f = fopen(filename, "rb");
     if (!f) return NULL;
@@ -123,18 +106,11 @@ matrix_z_t *matrix_LoadCSR_z(const char *filename)
     f = NULL;
     if (matrix_CheckCSR(M->n, M->nnz, M->colind, M->rowptr) != SPIKE_SUCCESS) goto fail;
     if (hdr[2] != 0) {
-        complex16 *inter = (complex16 *)xmalloc(sizeof(complex16) * M->nnz);
         for (uLong_t i = 0; i < M->nnz; ++i) {
-            M->aij[i].real = raw[i];
-            M->aij[i].imag = raw[M->nnz + i];
-            inter[i].real = raw[2 * i];
-            inter[i].imag = raw[2 * i + 1];
+            M->aij[i].real = planar ? raw[i] : raw[2 * i];
+            M->aij[i].imag = planar ? raw[M->nnz + i] : raw[2 * i + 1];
         }
-        long sp = dominance_score(M->n, M->rowptr, M->colind, M->aij);
-        long si = dominance_score(M->n, M->rowptr, M->colind, inter);
-        if (si > sp) { memcpy(M->aij, inter, sizeof(complex16) * M->nnz); M->planar_file = 0; }
-        else M->planar_file = 1;
-        free(inter);
+        M->planar_file = planar ? 1 : 0;
     }
     free(raw);
     matrix_ComputeBandwidth(M->n, M->colind, M->rowptr, &M->ku, &M->kl);
@@ -362,3 +338,5 @@ Error_t block_AssembleReducedRHS(double *g, integer_t P, integer_t ku, integer_t
         }
     return SPIKE_SUCCESS;
 }
+
+matrix_z_t *matrix_LoadCSR_z(const char *filename) { return matrix_LoadCSR_zl(filename, 1); }

@@ -55,7 +55,16 @@ typedef struct {
     int     reduced_iters;     /* block-Jacobi sweeps of the reduced system in the last solve */
     double  reduced_update;    /* last relative update of the reduced iteration             */
     double  max_rel_residual;  /* max over columns of ||b - A x|| / ||b|| (FP64, GPU)        */
-    double  ms_convert, ms_factor, ms_spikes, ms_reduced, ms_recover, ms_residual;
+    /* per-kernel times from HIP events on the handle's stream (milliseconds) */
+    double  ms_convert;        /* CSR -> blocks                                               */
+    double  ms_factor;         /* block LU (k_factor)                                         */
+    double  ms_rtop;           /* top block rows of the U inverse (k_rtop)                    */
+    double  ms_spikes;         /* V / W spike tips (k_sweep)                                  */
+    double  ms_red_setup;      /* reduced-system blocks (k_red_setup)                         */
+    double  ms_ysweep;         /* RHS tips, summed over refinement passes (k_sweep)          */
+    double  ms_reduced;        /* reduced-system iterations, summed                          */
+    double  ms_recover;        /* recovery sweeps, summed (k_recover)                        */
+    double  ms_residual;       /* FP64 residuals, summed (k_residual)                        */
     double  ms_factor_total;   /* spike_factor wall time                                     */
     double  ms_solve_total;    /* spike_solve_dev wall time                                  */
     size_t  device_bytes;      /* HBM held by the handle                                     */

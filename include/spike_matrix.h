@@ -72,7 +72,7 @@ typedef struct {
     integer_t   *colind;
     integer_t   *rowptr;
     complex16   *aij;
-    integer_t    planar_file;   /* 1 when the file held planar re/im arrays (generator_banded.py:134-141) */
+    integer_t    planar_file;   /* 1 when the file was read as planar re/im arrays (generator_banded.py:134-141) */
 } matrix_z_t;
 
 /* dense column-major panel: spike_matrix.h:62-72, aij[row + col*n] */
@@ -102,7 +102,8 @@ typedef struct {
 /* ---- CSR file IO (format: int32 n, int32 nnz, int32 dtype, aij[nnz], colind[nnz], rowptr[n+1]) */
 matrix_t   *matrix_LoadCSR(const char *filename);                          /* spike_matrix.c:11-51 */
 Error_t     matrix_SaveCSR(const char *filename, const matrix_t *M);       /* spike_matrix.c:56-85 */
-matrix_z_t *matrix_LoadCSR_z(const char *filename);                        /* complex: detects planar vs interleaved */
+matrix_z_t *matrix_LoadCSR_z(const char *filename);                        /* complex, planar values (generator_banded.py:134-141) */
+matrix_z_t *matrix_LoadCSR_zl(const char *filename, int planar);           /* complex, planar (1) or interleaved (0) values */
 Error_t     matrix_Deallocate(matrix_t *M);                                /* spike_matrix.c:265-273 */
 Error_t     matrix_z_Deallocate(matrix_z_t *M);
 

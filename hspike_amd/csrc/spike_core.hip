@@ -126,13 +126,13 @@ extern "C" int spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dt
     static const long blocks_per_part = [] {
         const char *v = getenv("HSPIKE_BLOCKS_PER_PART");
         const long x = v ? atol(v) : 0;
-        return x > 0 ? x : 24L;
+        return x > 0 ? x : 12L;
     }();
     const long rows_target = blocks_per_part * b;
     long P = n / rows_target;
     const long min_rows = std::max(2 * b, ku + kl);
     P = std::min(P, (long)n / std::max(1L, min_rows));
-    P = std::max(1L, std::min(P, 1024L));
+    P = std::max(1L, std::min(P, 8192L));
     return (int)P;
 }
 
@@ -177,6 +177,8 @@ struct Group {
     int waves_vw = 4, waves_y = 1;
     int ymax = 0;
     std::vector<uint64_t> solve_key;        // descriptors of the last solve chunk (reused when equal)
+    bool plan_uploaded = false;             // d_parts .. d_sysdev hold this plan
+    bool vw_uploaded = false;               // items_vw holds this plan's V/W items
     int red_last = 0;                       // buffer (0: red0, 1: red1) holding the last iterate
     std::vector<double> piv_host;
 };
@@ -190,6 +192,7 @@ struct Engine {
     int ngroups = 0;
     bool factored = false;
     DevBuf d_bw, d_bwchunks, d_bwsys;
+    std::vector<uint64_t> bw_key;           // systems described by d_bwchunks / d_bwsys
     DevBuf own_rowptr, own_colind, own_aij;   // host-matrix copies (spike_set_matrix_host)
     DevBuf stage_b, stage_x;                   // host solve staging
     spike_stats_t st;
@@ -280,10 +283,17 @@ int Engine::factor() {
             sd[i].rowptr = sys[i].d_rowptr;
             sd[i].colind = sys[i].d_colind;
             sd[i].n = sys[i].n;
-            for (int r = 0; r < sys[i].n; r += RES_ROWS) ch.push_back({i, r});
+            for (int r = 0; r < sys[i].n; r += BW_ROWS) ch.push_back({i, r});
         }
-        if ((e = upload(d_bwchunks, ch.data(), ch.size() * sizeof(RowChunk), s))) return e;
-        if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
+        std::vector<uint64_t> key;
+        for (int i = 0; i < nsys; ++i)
+            key.insert(key.end(), {(uint64_t)(uintptr_t)sys[i].d_rowptr, (uint64_t)(uintptr_t)sys[i].d_colind,
+                                   (uint64_t)sys[i].n});
+        if (key != bw_key) {
+            if ((e = upload(d_bwchunks, ch.data(), ch.size() * sizeof(RowChunk), s))) return e;
+            if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
+            bw_key = key;
+        }
         if ((e = d_bw.ensure(3 * sizeof(int) * nsys))) return e;
         HIPCK(hipMemsetAsync(d_bw.p, 0, 3 * sizeof(int) * nsys, s));
         if (!ch.empty())
@@ -327,12 +337,17 @@ int Engine::factor() {
         if (members.empty()) continue;
         if ((int)groups.size() <= ngroups) groups.emplace_back(new Group());
         Group &g = *groups[ngroups];
+        std::vector<PartDesc> old_parts;
+        old_parts.swap(g.parts);
+        const bool same_type = g.NT == nts[q] && g.tcode == tcode && g.sys == members;
+        std::vector<SysDev> old_sysdev;
+        old_sysdev.swap(g.sysdev);
         g.NT = nts[q];
         g.b = 16 * g.NT;
         g.tcode = tcode;
         g.sys = members;
-        g.parts.clear(); g.ifs.clear(); g.part_iface.clear(); g.chunks.clear(); g.chunk0.clear();
-        g.sysdev.clear(); g.sys_part0.clear();
+        g.ifs.clear(); g.part_iface.clear(); g.chunks.clear(); g.chunk0.clear();
+        g.sys_part0.clear();
         int64_t blk = 0;
         for (int gs = 0; gs < (int)members.size(); ++gs) {
             SysHost &S = sys[members[gs]];
@@ -369,9 +384,16 @@ int Engine::factor() {
             g.sysdev.push_back(sd);
         }
         g.chunk0.push_back((int)g.chunks.size());
-        g.solve_key.clear();
         g.nblk = blk;
         g.rows_pad = blk * g.b;
+        bool same = same_type && old_parts.size() == g.parts.size() && old_sysdev.size() == g.sysdev.size() &&
+                    !memcmp(old_parts.data(), g.parts.data(), g.parts.size() * sizeof(PartDesc));
+        for (size_t gs = 0; same && gs < g.sysdev.size(); ++gs)
+            same = old_sysdev[gs].rowptr == g.sysdev[gs].rowptr && old_sysdev[gs].colind == g.sysdev[gs].colind &&
+                   old_sysdev[gs].aij == g.sysdev[gs].aij;
+        if (same) g.sysdev.swap(old_sysdev);    // keep the solve fields of the uploaded copy
+        g.plan_uploaded = same && g.plan_uploaded;
+        if (!g.plan_uploaded) g.solve_key.clear();
         ++ngroups;
     }
 
@@ -379,12 +401,15 @@ int Engine::factor() {
         Group &g = *groups[gi];
         const size_t bb = (size_t)g.b * g.b;
         const size_t np = g.parts.size(), nif = g.ifs.size();
-        if ((e = upload(g.d_parts, g.parts.data(), np * sizeof(PartDesc), s))) return e;
-        if ((e = upload(g.d_ifs, g.ifs.data(), nif * sizeof(IfaceDesc), s))) return e;
-        if ((e = upload(g.d_part_iface, g.part_iface.data(), g.part_iface.size() * sizeof(int), s))) return e;
-        if ((e = upload(g.d_chunks, g.chunks.data(), g.chunks.size() * sizeof(RowChunk), s))) return e;
-        if ((e = upload(g.d_chunk0, g.chunk0.data(), g.chunk0.size() * sizeof(int), s))) return e;
-        if ((e = upload(g.d_sysdev, g.sysdev.data(), g.sysdev.size() * sizeof(SysDev), s))) return e;
+        if (!g.plan_uploaded) {
+            if ((e = upload(g.d_parts, g.parts.data(), np * sizeof(PartDesc), s))) return e;
+            if ((e = upload(g.d_ifs, g.ifs.data(), nif * sizeof(IfaceDesc), s))) return e;
+            if ((e = upload(g.d_part_iface, g.part_iface.data(), g.part_iface.size() * sizeof(int), s))) return e;
+            if ((e = upload(g.d_chunks, g.chunks.data(), g.chunks.size() * sizeof(RowChunk), s))) return e;
+            if ((e = upload(g.d_chunk0, g.chunk0.data(), g.chunk0.size() * sizeof(int), s))) return e;
+            if ((e = upload(g.d_sysdev, g.sysdev.data(), g.sysdev.size() * sizeof(SysDev), s))) return e;
+            g.vw_uploaded = false;
+        }
         DevBuf *tb[] = {&g.D, &g.U, &g.C, &g.R};
         for (auto b : tb)
             if ((e = b->ensure(g.nblk * bb * ts))) return e;
@@ -403,6 +428,7 @@ int Engine::factor() {
                 (e = fill_zero<T>(g.Ct, np * bb, s)))
                 return e;
         });
+        g.plan_uploaded = true;
     }
     HIPCK(hipEventRecord(ev[2], s));
     for (int gi = 0; gi < ngroups; ++gi) {
@@ -441,6 +467,20 @@ int Engine::factor() {
     // ---- V / W spike tips (partitions of systems with P > 1)
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
+        if (g.vw_uploaded) {
+            if (g.n_items_vw) {
+                T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+                    typedef typename tr<T>::H H;
+                    hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_vw), dim3(256), 0, s, g.items_vw.as<SweepItem>(),
+                                       g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
+                                       (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
+                                       (const T *)g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
+                                       (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+                }));
+                HIPCK(hipGetLastError());
+            }
+            continue;
+        }
         std::vector<SweepItem> items;
         const int cpi = 64;
         for (int q = 0; q < (int)g.parts.size(); ++q) {
@@ -455,6 +495,7 @@ int Engine::factor() {
                 for (int c = 0; c < d.kl; c += cpi) { it.kind = ITEM_W; it.col0 = c; it.ncols = std::min(cpi, d.kl - c); items.push_back(it); }
         }
         g.n_items_vw = (int)items.size();
+        g.vw_uploaded = true;
         if (items.empty()) continue;
         if ((e = upload(g.items_vw, items.data(), items.size() * sizeof(SweepItem), s))) return e;
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
@@ -472,17 +513,18 @@ int Engine::factor() {
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         if (g.ifs.empty()) continue;
-        const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
         if (dtype == SPIKE_DTYPE_Z) {
-            HIPCK(hipFuncSetAttribute((const void *)k_red_setup<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            hipLaunchKernelGGL(k_red_setup<zd>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s, g.d_ifs.as<IfaceDesc>(),
-                               (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(), g.b, g.status.as<int>() + 1,
-                               (const int *)nullptr);
+            NT_SWITCH(g.NT, {
+                hipLaunchKernelGGL((k_red_setup_mfma<zd, NT>), dim3((unsigned)g.ifs.size()), dim3(NT * 64), 0, s,
+                                   g.d_ifs.as<IfaceDesc>(), (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(),
+                                   g.status.as<int>() + 1, (const int *)nullptr);
+            });
         } else {
-            HIPCK(hipFuncSetAttribute((const void *)k_red_setup<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            hipLaunchKernelGGL(k_red_setup<double>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s,
-                               g.d_ifs.as<IfaceDesc>(), (const double *)g.vb.p, (const double *)g.wt.p, g.T.as<double>(),
-                               g.b, g.status.as<int>() + 1, (const int *)nullptr);
+            NT_SWITCH(g.NT, {
+                hipLaunchKernelGGL((k_red_setup_mfma<double, NT>), dim3((unsigned)g.ifs.size()), dim3(NT * 64), 0, s,
+                                   g.d_ifs.as<IfaceDesc>(), (const double *)g.vb.p, (const double *)g.wt.p,
+                                   g.T.as<double>(), g.status.as<int>() + 1, (const int *)nullptr);
+            });
         }
         HIPCK(hipGetLastError());
     }
@@ -501,6 +543,25 @@ int Engine::factor() {
                 set_err("zero or non-finite pivot in partition %s%ld", "", (long)q);
                 return SPIKE_ERR_SINGULAR;
             }
+        if (stv[1] && !g.ifs.empty()) {
+            // a 4x4 panel pivot of the MFMA inversion was zero: redo T_i with full partial pivoting
+            const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
+            HIPCK(hipMemsetAsync(g.status.p, 0, 4 * sizeof(int), s));
+            if (dtype == SPIKE_DTYPE_Z) {
+                HIPCK(hipFuncSetAttribute((const void *)k_red_setup<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                hipLaunchKernelGGL(k_red_setup<zd>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s, g.d_ifs.as<IfaceDesc>(),
+                                   (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(), g.b, g.status.as<int>() + 1,
+                                   (const int *)nullptr);
+            } else {
+                HIPCK(hipFuncSetAttribute((const void *)k_red_setup<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+                hipLaunchKernelGGL(k_red_setup<double>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s,
+                                   g.d_ifs.as<IfaceDesc>(), (const double *)g.vb.p, (const double *)g.wt.p, g.T.as<double>(),
+                                   g.b, g.status.as<int>() + 1, (const int *)nullptr);
+            }
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(stv, g.status.p, sizeof stv, hipMemcpyDeviceToHost, s));
+            HIPCK(hipStreamSynchronize(s));
+        }
         if (stv[1]) { set_err("singular reduced-system block%s", ""); return SPIKE_ERR_SINGULAR; }
     }
     factored = true;
@@ -636,30 +697,32 @@ int Engine::solve_chunk(int c0, int ymax) {
             std::vector<int> ract = active[gi];
             if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
             DevBuf *xo = &g.red0, *xn = &g.red1;
-            const size_t smem = (size_t)5 * g.b * 16 * hs;
-            if (dtype == SPIKE_DTYPE_Z)
-                HIPCK(hipFuncSetAttribute((const void *)k_red_jacobi<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            else
-                HIPCK(hipFuncSetAttribute((const void *)k_red_jacobi<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
             std::vector<double> conv(nif * ncc * 2);
             std::vector<double> dmax(ngs), vmax(ngs);
             int iters = 0;
             const int maxit = std::max(2, o.reduced_maxit);
             while (iters < maxit) {
-                for (int rep = 0; rep < (iters == 0 ? 2 : 1); ++rep) {
-                    if (dtype == SPIKE_DTYPE_Z)
-                        hipLaunchKernelGGL(k_red_jacobi<zd>, dim3((unsigned)(nif * ncc)), dim3(256), smem, s,
-                                           g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p, (const zd *)g.vb.p,
-                                           (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p, (const zd *)g.yb.p,
-                                           (const zd *)g.T.p, (const zd *)xo->p, xn->as<zd>(), g.conv.as<double>(),
-                                           g.b, ymax, ymax, g.d_red_active.as<int>());
-                    else
-                        hipLaunchKernelGGL(k_red_jacobi<double>, dim3((unsigned)(nif * ncc)), dim3(256), smem, s,
-                                           g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p, (const double *)g.vb.p,
-                                           (const double *)g.wt.p, (const double *)g.wb.p, (const double *)g.yt.p,
-                                           (const double *)g.yb.p, (const double *)g.T.p, (const double *)xo->p,
-                                           xn->as<double>(), g.conv.as<double>(), g.b, ymax, ymax,
-                                           g.d_red_active.as<int>());
+                // two sweeps per convergence check (the first sweep from zero is the truncated
+                // solution; the second measures the coupling); systems stop once converged
+                for (int rep = 0; rep < 2 && iters < maxit; ++rep) {
+                    if (dtype == SPIKE_DTYPE_Z) {
+                        NT_SWITCH(g.NT, {
+                            hipLaunchKernelGGL((k_red_jacobi<zd, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
+                                               g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p, (const zd *)g.vb.p,
+                                               (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p,
+                                               (const zd *)g.yb.p, (const zd *)g.T.p, (const zd *)xo->p, xn->as<zd>(),
+                                               g.conv.as<double>(), ymax, ymax, g.d_red_active.as<int>());
+                        });
+                    } else {
+                        NT_SWITCH(g.NT, {
+                            hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
+                                               g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
+                                               (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
+                                               (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
+                                               (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
+                                               ymax, g.d_red_active.as<int>());
+                        });
+                    }
                     HIPCK(hipGetLastError());
                     std::swap(xo, xn);
                     ++iters;

@@ -67,7 +67,14 @@ struct DevBuf {
             return SPIKE_ERR_OOM;
         }
         bytes = need;
+        if (poison_new()) (void)hipMemset(p, 0xff, need);   // NaN in every float type
         return 0;
+    }
+    // HSPIKE_POISON=1: fill new device buffers with NaN bytes, so any read of memory no kernel
+    // wrote shows up in the results (debug aid)
+    static bool poison_new() {
+        static const bool on = [] { const char *v = getenv("HSPIKE_POISON"); return v && *v == '1'; }();
+        return on;
     }
     template <typename X> X *as() const { return (X *)p; }
 };
@@ -174,7 +181,7 @@ struct Group {
     DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
     DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
     int n_items_vw = 0, n_items_y = 0, n_items_r = 0;
-    int waves_vw = 4, waves_y = 1;
+    int ct_vw = 4, ct_y = 1;                // column tiles per sweep workgroup (V/W items, RHS items)
     int ymax = 0;
     std::vector<uint64_t> solve_key;        // descriptors of the last solve chunk (reused when equal)
     bool plan_uploaded = false;             // d_parts .. d_sysdev hold this plan
@@ -248,6 +255,44 @@ struct Engine {
     case 1: { typedef double T; BODY; } break;               \
     case 2: { typedef zf T; BODY; } break;                   \
     case 3: { typedef zd T; BODY; } break;                   \
+    default: break;                                          \
+    }
+
+// Column tiles per sweep workgroup: CT * NT * sizeof(T) <= 64 keeps the two LDS exchange panels
+// (2 * 16*CT * b elements) within 32 KB and the per-lane registers (CT accumulator tiles per
+// product, NT-deep operand slices) low enough for two or more waves per SIMD.
+constexpr bool sweep_fits(int tbytes, int nt, int ct) { return ct * nt * tbytes <= 64; }
+int ct_max(int tcode, int NT) {
+    int ct = 4;
+    while (ct > 1 && !sweep_fits((int)type_bytes(tcode), NT, ct)) ct >>= 1;
+    return ct;
+}
+int ct_for(int ncols, int ctmax) { return std::min(ctmax, ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 4)); }
+
+template <typename T, int NT, int CT>
+void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
+                  const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
+                  typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, int ymax,
+                  const int *active) {
+    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(tr<T>::C && NT > 4))
+        hipLaunchKernelGGL((k_sweep<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
+                           vb, wt, wb, yt, yb, ymax, active);
+}
+
+template <typename T, int NT, int CT>
+void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
+                    const T *U, const T *Bt, const T *Ct, const typename tr<T>::H *red, const int *part_iface, int ymax,
+                    T *scratch, int accumulate, const int *active) {
+    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(tr<T>::C && NT > 4))
+        hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, Bt, Ct, red,
+                           part_iface, ymax, (Q4<T> *)scratch, accumulate, active);
+}
+
+#define CT_SWITCH(CTV, BODY)                                 \
+    switch (CTV) {                                           \
+    case 1: { constexpr int CT = 1; BODY; } break;           \
+    case 2: { constexpr int CT = 2; BODY; } break;           \
+    case 4: { constexpr int CT = 4; BODY; } break;           \
     default: break;                                          \
     }
 
@@ -469,20 +514,20 @@ int Engine::factor() {
         Group &g = *groups[gi];
         if (g.vw_uploaded) {
             if (g.n_items_vw) {
-                T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+                T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
                     typedef typename tr<T>::H H;
-                    hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_vw), dim3(256), 0, s, g.items_vw.as<SweepItem>(),
-                                       g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
-                                       (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
-                                       (const T *)g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
-                                       (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
-                }));
+                    (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
+                                            g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
+                                            g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
+                                            (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+                })));
                 HIPCK(hipGetLastError());
             }
             continue;
         }
         std::vector<SweepItem> items;
-        const int cpi = 64;
+        g.ct_vw = ct_max(g.tcode, g.NT);
+        const int cpi = 16 * g.ct_vw;
         for (int q = 0; q < (int)g.parts.size(); ++q) {
             const PartDesc &d = g.parts[q];
             if (d.P == 1) continue;
@@ -498,14 +543,13 @@ int Engine::factor() {
         g.vw_uploaded = true;
         if (items.empty()) continue;
         if ((e = upload(g.items_vw, items.data(), items.size() * sizeof(SweepItem), s))) return e;
-        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+        T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
             typedef typename tr<T>::H H;
-            hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_vw), dim3(256), 0, s, g.items_vw.as<SweepItem>(),
-                               g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
-                               (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
-                               (const T *)g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
-                               (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
-        }));
+            (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
+                                    g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
+                                    g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
+                                    (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+        })));
         HIPCK(hipGetLastError());
     }
     HIPCK(hipEventRecord(ev[6], s));
@@ -592,7 +636,7 @@ int Engine::solve_chunk(int c0, int ymax) {
         if ((e = g.yt.ensure(np * g.b * ymax * hs)) || (e = g.yb.ensure(np * g.b * ymax * hs)) ||
             (e = g.red0.ensure(std::max<size_t>(nif, 1) * 2 * g.b * ymax * hs)) ||
             (e = g.red1.ensure(std::max<size_t>(nif, 1) * 2 * g.b * ymax * hs)) ||
-            (e = g.scratch.ensure((size_t)g.rows_pad * ymax * ts)) ||
+            (e = g.scratch.ensure((size_t)g.rows_pad * 16 * ceil_div(ymax, 16) * ts)) ||
             (e = g.partials.ensure(g.chunks.size() * ymax * 2 * sizeof(double))) ||
             (e = g.norms.ensure(ngs * ymax * 2 * sizeof(double))) ||
             (e = g.conv.ensure(std::max<size_t>(nif, 1) * ceil_div(ymax, 16) * 2 * sizeof(double))))
@@ -631,8 +675,8 @@ int Engine::solve_chunk(int c0, int ymax) {
         }
         if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
         // items: Y sweeps and recoveries, from b (first pass) and from r (refinement passes)
-        g.waves_y = ymax <= 16 ? 1 : (ymax <= 32 ? 2 : 4);
-        const int cpi = 16 * g.waves_y;
+        g.ct_y = ct_for(ymax, ct_max(g.tcode, g.NT));
+        const int cpi = 16 * g.ct_y;
         std::vector<SweepItem> yb_, yr_, rb_, rr_;
         for (int q = 0; q < (int)np; ++q) {
             const PartDesc &d = g.parts[q];
@@ -673,14 +717,13 @@ int Engine::solve_chunk(int c0, int ymax) {
             Group &g = *groups[gi];
             if (!g.n_items_y) continue;
             const DevBuf &items = it == 0 ? g.items_yb : g.items_yr;
-            T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+            T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
-                hipLaunchKernelGGL((k_sweep<T, NT>), dim3(g.n_items_y), dim3(64 * g.waves_y), 0, s, items.as<SweepItem>(),
-                                   g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
-                                   (const T *)g.U.as<T>(), (const T *)g.R.as<T>(), (const T *)g.Bt.as<T>(),
-                                   (const T *)g.Ct.as<T>(), (H *)nullptr, (H *)nullptr, (H *)nullptr, (H *)nullptr,
-                                   g.yt.as<H>(), g.yb.as<H>(), g.ymax, g.d_active.as<int>());
-            }));
+                (launch_sweep<T, NT, CT>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
+                                        g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
+                                        (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(), g.ymax,
+                                        g.d_active.as<int>());
+            })));
             HIPCK(hipGetLastError());
         }
         HIPCK(hipEventRecord(ev[9], s));
@@ -759,14 +802,13 @@ int Engine::solve_chunk(int c0, int ymax) {
             Group &g = *groups[gi];
             if (!g.n_items_r) continue;
             const DevBuf &items = it == 0 ? g.items_rb : g.items_rr;
-            T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+            T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
-                hipLaunchKernelGGL((k_recover<T, NT>), dim3(g.n_items_r), dim3(64 * g.waves_y), 0, s, items.as<SweepItem>(),
-                                   g.d_parts.as<PartDesc>(), (const T *)g.D.as<T>(), (const T *)g.C.as<T>(),
-                                   (const T *)g.U.as<T>(), (const T *)g.Bt.as<T>(), (const T *)g.Ct.as<T>(),
-                                   (const H *)redcur[gi]->p, g.d_part_iface.as<int>(), g.ymax, g.scratch.as<T>(),
-                                   (int64_t)g.rows_pad, it > 0 ? 1 : 0, g.d_active.as<int>());
-            }));
+                (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
+                                          g.C.as<T>(), g.U.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (const H *)redcur[gi]->p,
+                                          g.d_part_iface.as<int>(), g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0,
+                                          g.d_active.as<int>());
+            })));
             HIPCK(hipGetLastError());
         }
         HIPCK(hipEventRecord(ev[11], s));

@@ -98,6 +98,16 @@ template <typename T> __device__ __forceinline__ int trow(int g, int e) {
     return row_major_groups<T>() ? (4 * g + e) : (g + 4 * e);
 }
 
+// ---------------------------------------------------------------- block storage
+// Element (r, c) of a b x b factor block (D/Dinv, B/Uq, C, R) is stored at r * b + acol(c):
+// rows are contiguous and the columns inside each 16-column group are permuted so that the
+// four left-operand values a lane feeds to MFMA steps e = 0..3 of one K tile (columns
+// 16 t2 + trow(g, e)) are adjacent -- one Q4 (16-64 byte) load per lane and K tile.
+template <typename T> __device__ __forceinline__ int acol(int c) {
+    return row_major_groups<T>() ? c : ((c & ~15) | ((c & 3) << 2) | ((c >> 2) & 3));
+}
+template <typename T> struct alignas(4 * sizeof(T)) Q4 { T v[4]; };
+
 // 16 x 16 accumulator tile distributed over a wave64 (4 elements per lane)
 template <typename T, bool C = tr<T>::C> struct Tile;
 
@@ -123,6 +133,33 @@ template <typename T> struct Tile<T, true> {
         im = mfma_step(a.im, b.re, im);
     }
 };
+
+// Extra wait states between the last MFMA of an f64 accumulation chain and the first read of
+// its result.  On gfx950 the wait states the compiler inserts before reading a
+// v_mfma_f64_16x16x4f64 result were measured to be too few when the wave issues alone on its
+// SIMD (the tail of a launch): the read returned partly stale accumulator values, so the last
+// workgroup of a sweep produced wrong tips.  Sixteen more wait states remove the error; the asm
+// takes the accumulator as an operand, so it is placed after the chain.
+template <typename T> __device__ __forceinline__ void tile_settle(Tile<T> &x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(typename tr<T>::R) == 8) {
+        if constexpr (tr<T>::C) {
+            asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.re), "+a"(x.im));
+        } else {
+            asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.v));
+        }
+    }
+#else
+    (void)x;
+#endif
+}
+
+template <typename T> __device__ __forceinline__ Q4<T> tile_q(const Tile<T> &x) {
+    Q4<T> r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r.v[e] = x.get(e);
+    return r;
+}
 
 // ---------------------------------------------------------------- block descriptors
 // One SPIKE partition of one system.  Blocks of b = 16*NT rows; the partition's last block is

@@ -127,6 +127,33 @@ def test_fp64_parity_random(hs, oracle_mod, n, ku, kl, fill, P, nrhs):
     np.testing.assert_allclose(hs.residual(A, Xp, B), oracle_mod.residual(to_o(oracle_mod, A), Xp, B), rtol=1e-10)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_single_pass_tail_partition_and_determinism(hs, oracle_mod, prec):
+    """64 partitions of 769 rows and a last one of 829 (the C2 layout at 1/20 size): the last
+    partition's workgroup runs alone at the end of every sweep.  FP64 must reach the bar in one
+    pass (no refinement) with every partition's tips equal to the oracle's, and two solves must
+    agree bit for bit (fixed-order arithmetic, no races)."""
+    n, k, P, nrhs = 64 * 769 + 60, 64, 64, 16
+    A = hs.gen_banded(n, k, k, fill=0.15, dom=(1.05, 1.05), diag_floor=1.0, seed=1000 + n + k)
+    B = hs.gen_dense(n, nrhs, seed=7)
+    runs = []
+    for _ in range(2):
+        s = hs.Solver(prec, partitions=P, max_refine=0 if prec == "fp64" else 8, tol=1e-12)
+        s.set_matrix(A)
+        X = s.solve(B)
+        runs.append((X, s.tips(nrhs), s.stats()))
+        s.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for key in ("Vt", "Vb", "Wt", "Wb", "yt", "yb"):
+        np.testing.assert_array_equal(runs[0][1][key], runs[1][1][key], err_msg=key)
+    if prec == "fp64":
+        assert runs[0][2]["refine_iters"] == 0 and runs[0][2]["max_rel_residual"] <= 1e-14
+        Xo, info = oracle_mod.spike_solve(to_o(oracle_mod, A), B, P, want_reduced=False)
+        for key in ("Vt", "Vb", "Wt", "Wb", "yt", "yb"):
+            np.testing.assert_allclose(runs[0][1][key], info[key], atol=1e-12, err_msg=key)
+        assert rel_err(runs[0][0], Xo) <= 1e-12
+
+
 @pytest.mark.parametrize("n,k,fill,dom,nrhs", [
     (20000, 64, 0.15, (1.05, 1.05), 16),
     (30000, 128, 0.2, (1.01, 2.0), 8),    # C3 shape at a reduced size

@@ -44,8 +44,9 @@ SEED_A, SEED_B = 20261202, 7
 
 
 def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
-    """Algorithmic flops and HBM bytes of one launch of a stage (counting formulas of the
-    SURVEY's measurement section; implementation overheads never enter)."""
+    """Algorithmic flops and HBM bytes of one launch of a stage: the counting formulas of the
+    benchmark definition (band read/written once, RHS and solution panels once per pass);
+    implementation overheads never enter."""
     if name == "factor":      # banded LU of all partitions
         return 2.0 * n * kl * ku, 2.0 * n * (kl + ku + 1) * s
     if name == "spikes":      # V/W spike tips: m_i = ku [i < P-1] + kl [i > 0] columns per partition
@@ -60,6 +61,19 @@ def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
     raise KeyError(name)
 
 
+def measured_traffic(config, kernel):
+    """HBM bytes per dispatch of `kernel` on `config` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated); None when that kernel/workload was not measured."""
+    path = os.path.join(ROOT, "profiles", "spike_%s_mixed_hbm_traffic.md" % config)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 5 and parts[0] == "traffic_bytes" and parts[1] == config and parts[2] == kernel:
+            return int(parts[4])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +82,18 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison runs")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only run a plain device copy of 256 MiB three times (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.probe_copy:
+        import torch
+        x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print("[bench] copy probe: 3 x (256 MiB read + 256 MiB written)", file=sys.stderr)
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU SPIKE (one partition range per "
               "GPU, tip exchange over xGMI) is a design note, not a benchmark here", file=sys.stderr)
@@ -164,8 +189,15 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": round(byte_rate, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(byte_rate / HBM_PEAK_GBS, 4)}
-    roof.update({"traffic": None, "kernel": {"factor": "k_factor", "spikes": "k_sweep (V/W)", "recover": "k_recover",
-                                             "ysweep": "k_sweep (RHS)", "residual": "k_residual"}[dom],
+    # instantiation the library launches for this shape (FP32 factors, b = 16 NT, column tiles per
+    # sweep workgroup limited to CT * NT <= 16)
+    nt = mixed["last"]["b"] // 16
+    ct_vw = min(4, 16 // nt)
+    ct_y = min(ct_vw, 1 if min(nrhs, 64) <= 16 else (2 if min(nrhs, 64) <= 32 else 4))
+    kname = {"factor": "k_factor<float,%d>" % nt, "spikes": "k_sweep<float,%d,%d>" % (nt, ct_vw),
+             "recover": "k_recover<float,%d,%d>" % (nt, ct_y), "ysweep": "k_sweep<float,%d,%d>" % (nt, ct_y),
+             "residual": "k_residual<double>"}[dom]
+    roof.update({"traffic": measured_traffic(args.config, kname), "kernel": kname,
                  "stage": dom, "algorithmic_flops": F, "algorithmic_bytes": Bytes, "ms_per_launch": round(per_launch_ms, 4)})
 
     # ---- CPU baseline: the oracle on the host cores, same matrix and RHS, configured partitions

@@ -12,6 +12,8 @@ Workloads (synthetic, seeded):
   c3           n = 2*10^6, half-bandwidth 128, 20% fill, per-row dominance U[1.01, 2.0], 64 RHS,
                refinement to 1e-12
   c1           n = 10^4 pentadiagonal, a_ii = sum|a_ij| + 1, 4 partitions, 1 RHS
+  c4           4096 independent systems in one batched solve: n log-uniform [10^3, 2*10^5],
+               half-bandwidth uniform in {4, 8, 16, 32, 64}, 30% fill, dominance 1.1, nrhs 1..8
 The CPU baseline is the oracle (oracle/, a C restatement of the reference's SPIKE flow with a
 banded partial-pivot LU in place of SuperLU_MT) timed on the host cores on the same matrix.
 
@@ -41,6 +43,17 @@ CONFIGS = {
                desc="pentadiagonal n=1e4, a_ii = sum|a_ij| + 1, 1 RHS"),
 }
 SEED_A, SEED_B = 20261202, 7
+C4 = dict(nsys=4096, n_lo=1e3, n_hi=2e5, ks=(4, 8, 16, 32, 64), fill=0.30, dom=1.1, nrhs_hi=8, tol=1e-10,
+          seed=20261204)
+
+
+def c4_systems(nsys, seed):
+    """The C4 batch: (n, k, nrhs, matrix seed, RHS seed) per system, drawn from `seed`."""
+    rng = np.random.RandomState(seed)
+    ns = np.exp(rng.uniform(np.log(C4["n_lo"]), np.log(C4["n_hi"]), nsys)).astype(np.int64)
+    ks = rng.choice(np.array(C4["ks"]), nsys)
+    nr = rng.randint(1, C4["nrhs_hi"] + 1, nsys)
+    return [(int(ns[i]), int(ks[i]), int(nr[i]), seed * 7919 + i, seed + 100003 * i) for i in range(nsys)]
 
 
 def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
@@ -79,7 +92,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--nsys", type=int, default=C4["nsys"], help="c4: number of systems in the batch")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison runs")
     ap.add_argument("--probe-copy", action="store_true",
@@ -102,6 +116,8 @@ def main():
     import torch
     import hspike_amd as hs
 
+    if args.config == "c4":
+        return bench_batch(args, torch, hs)
     cfg = CONFIGS[args.config]
     n, k, nrhs = cfg["n"], cfg["k"], cfg["nrhs"]
     t_gen = time.time()
@@ -238,6 +254,111 @@ def main():
         "stage_ms": {**{k: round(v, 4) for k, v in avg.items()}, **{k: round(v, 4) for k, v in avg_other.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_batch(args, torch, hs):
+    """C4: one batched factor + solve of many independent systems per step (spike_batch_solve_dev)."""
+    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    spec = c4_systems(args.nsys, C4["seed"])
+    t_gen = time.time()
+    systems, host = [], []
+    for (n, k, nrhs, sa, sb) in spec:
+        A = hs.gen_banded(n, k, k, fill=C4["fill"], dom=(C4["dom"], C4["dom"]), diag_floor=1.0, seed=sa)
+        B = hs.gen_dense(n, nrhs, seed=sb)
+        systems.append(dict(n=n, nrhs=nrhs, rowptr=torch.from_numpy(A.rowptr).to(dev),
+                            colind=torch.from_numpy(A.colind).to(dev), aij=torch.from_numpy(A.aij).to(dev),
+                            b=torch.from_numpy(np.ascontiguousarray(B.T)).to(dev),
+                            x=torch.zeros((nrhs, n), dtype=torch.float64, device=dev)))
+        host.append((A, B, k))
+        if len(host) % 512 == 0:
+            log("c4: %d of %d systems generated and uploaded" % (len(host), len(spec)))
+    torch.cuda.synchronize()
+    rows = sum(s["n"] for s in systems)
+    nnz = sum(A.nnz for A, _, _ in host)
+    rhs = sum(s["nrhs"] for s in systems)
+    log("generated c4: %d systems, %d rows, nnz %d, %d RHS in %.1f s" % (len(systems), rows, nnz, rhs, time.time() - t_gen))
+
+    def run(precision, steps, warmup):
+        bt = hs.Batch(precision, tol=C4["tol"])
+        for _ in range(warmup):
+            bt.solve_dev(systems)
+        torch.cuda.synchronize()
+        lat, rels, sts = [], None, []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            rels, st = bt.solve_dev(systems)
+            lat.append((time.perf_counter() - ts) * 1e3)
+            sts.append(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bt.close()
+        return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts)
+
+    mixed = run("mixed", args.steps, args.warmup)
+    log("mixed: %.2f ms/step, max rel residual %.3e" % (mixed["wall"] / args.steps * 1e3, mixed["rel"].max()))
+    f64 = None
+    if not args.no_fp64:
+        r = run("fp64", max(2, args.steps // 2), 1)
+        f64 = {"rhs_per_s": round(len(r["lat"]) * rhs / r["wall"], 2), "ms_per_step": round(r["wall"] / len(r["lat"]) * 1e3, 3),
+               "max_rel_residual": float(r["rel"].max())}
+        log("fp64: %s" % f64)
+    rhs_s = args.steps * rhs / mixed["wall"]
+    keys = {"factor": "ms_factor", "spikes": "ms_spikes", "recover": "ms_recover", "ysweep": "ms_ysweep",
+            "residual": "ms_residual"}
+    avg = {k: float(np.mean([st[v] for st in mixed["stats"]])) for k, v in keys.items()}
+    passes = max(1, max(st["refine_iters"] for st in mixed["stats"]) + 1)
+    dom = max(avg, key=avg.get)
+    launches = passes if dom in ("recover", "ysweep", "residual") else 1
+    F = Bts = 0.0
+    for (A, B, k) in host:
+        P = hs.autotune_partitions(A.n, k, k, B.shape[1], precision=hs.SPIKE_PREC_MIXED)
+        f, b = algorithmic_counts(dom, A.n, A.nnz, k, k, B.shape[1], P, 4)
+        F += f
+        Bts += b
+    per_launch_ms = avg[dom] / launches
+    byte_rate = Bts / (per_launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(byte_rate, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(byte_rate / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": {"factor": "k_factor", "spikes": "k_sweep (V/W)", "recover": "k_recover", "ysweep": "k_sweep (RHS)",
+                       "residual": "k_residual"}[dom] + " (all block sizes of the batch)",
+            "stage": dom, "algorithmic_flops": F, "algorithmic_bytes": Bts, "ms_per_launch": round(per_launch_ms, 4)}
+    cpu = None
+    if not args.no_cpu:      # the oracle on a bounded sample of the batch, scaled to RHS/s
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+        threads = min(16, os.cpu_count() or 1)
+        t0, done, cnt = time.perf_counter(), 0, 0
+        for (A, B, k) in host:
+            oracle.spike_solve(oracle.CSR(A.n, A.rowptr, A.colind, A.aij), B, max(1, min(4, A.n // (4 * k + 4))),
+                               nthreads=threads, want_reduced=False)
+            done += B.shape[1]
+            cnt += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(done / t_cpu, 3), "unit": "RHS/s", "cores": threads, "kind": "port",
+               "sample": "first %d of %d systems (%d RHS), oracle SPIKE FP64 with up to 4 partitions, %.1f s" %
+                         (cnt, len(host), done, t_cpu)}
+    lat = mixed["lat"]
+    line = {
+        "metric": METRIC, "value": round(rhs_s, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(mixed["wall"] / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "C4: %d independent banded systems, n log-uniform [1e3, 2e5], half-bandwidth in "
+                               "{4,8,16,32,64}, 30%% fill, dominance 1.1, nrhs 1-8" % len(systems),
+                   "systems": len(systems), "rows": rows, "nnz": nnz, "nrhs_total": rhs, "tol": C4["tol"],
+                   "seed": C4["seed"], "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "accuracy": {"max_rel_residual": float(mixed["rel"].max())},
+        "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
+        "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     return 0

@@ -288,6 +288,22 @@ void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const Par
                            part_iface, ymax, (Q4<T> *)scratch, accumulate, active);
 }
 
+// FP64 residual with TW column lanes per row: the narrowest of 4, 8, 16 covering the chunk's
+// columns (all chunks share the partials stride `ymax`)
+int launch_residual(int dtype, int mcols, int ymax, unsigned nchunks, hipStream_t s, const RowChunk *chunks,
+                    const SysDev *sys, double *partials, const int *active) {
+    const int tw = mcols <= 4 ? 4 : (mcols <= 8 ? 8 : 16);
+#define RES_LAUNCH(H, TW) hipLaunchKernelGGL((k_residual<H, TW>), dim3(nchunks), dim3(256), 0, s, chunks, sys, partials, ymax, active)
+    if (dtype == SPIKE_DTYPE_Z) {
+        if (tw == 4) RES_LAUNCH(zd, 4); else if (tw == 8) RES_LAUNCH(zd, 8); else RES_LAUNCH(zd, 16);
+    } else {
+        if (tw == 4) RES_LAUNCH(double, 4); else if (tw == 8) RES_LAUNCH(double, 8); else RES_LAUNCH(double, 16);
+    }
+#undef RES_LAUNCH
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
 #define CT_SWITCH(CTV, BODY)                                 \
     switch (CTV) {                                           \
     case 1: { constexpr int CT = 1; BODY; } break;           \
@@ -815,13 +831,9 @@ int Engine::solve_chunk(int c0, int ymax) {
         // ---- FP64 residual of every column
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
-            if (dtype == SPIKE_DTYPE_Z)
-                hipLaunchKernelGGL(k_residual<zd>, dim3((unsigned)g.chunks.size()), dim3(256), 0, s, g.d_chunks.as<RowChunk>(),
-                                   g.d_sysdev.as<SysDev>(), g.partials.as<double>(), ymax, g.d_active.as<int>());
-            else
-                hipLaunchKernelGGL(k_residual<double>, dim3((unsigned)g.chunks.size()), dim3(256), 0, s,
-                                   g.d_chunks.as<RowChunk>(), g.d_sysdev.as<SysDev>(), g.partials.as<double>(), ymax,
-                                   g.d_active.as<int>());
+            if ((e = launch_residual(dtype, ymax, ymax, (unsigned)g.chunks.size(), s, g.d_chunks.as<RowChunk>(),
+                                     g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.d_active.as<int>())))
+                return e;
             hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(g.sys.size() * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
                                g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.norms.as<double>(), ymax);
             HIPCK(hipGetLastError());
@@ -1244,12 +1256,9 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
         if ((e = upload(dch, ch.data(), ch.size() * sizeof(RowChunk), 0)) || (e = upload(dsys, &sd, sizeof sd, 0)) ||
             (e = upload(dc0, c0s, sizeof c0s, 0)) || (e = dpart.ensure((size_t)nch * 64 * 2 * 8)) || (e = dnorm.ensure(64 * 2 * 8)))
             return e;
-        if (dtype == SPIKE_DTYPE_Z)
-            hipLaunchKernelGGL(k_residual<zd>, dim3(nch), dim3(256), 0, 0, dch.as<RowChunk>(), dsys.as<SysDev>(),
-                               dpart.as<double>(), 64, (const int *)nullptr);
-        else
-            hipLaunchKernelGGL(k_residual<double>, dim3(nch), dim3(256), 0, 0, dch.as<RowChunk>(), dsys.as<SysDev>(),
-                               dpart.as<double>(), 64, (const int *)nullptr);
+        if ((e = launch_residual(dtype, m, 64, (unsigned)nch, 0, dch.as<RowChunk>(), dsys.as<SysDev>(), dpart.as<double>(),
+                                 (const int *)nullptr)))
+            return e;
         hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),
                            dnorm.as<double>(), 64);
         HIPCK(hipGetLastError());

@@ -134,20 +134,19 @@ template <typename T> struct Tile<T, true> {
     }
 };
 
-// Extra wait states between the last MFMA of an f64 accumulation chain and the first read of
-// its result.  On gfx950 the wait states the compiler inserts before reading a
-// v_mfma_f64_16x16x4f64 result were measured to be too few when the wave issues alone on its
-// SIMD (the tail of a launch): the read returned partly stale accumulator values, so the last
-// workgroup of a sweep produced wrong tips.  Sixteen more wait states remove the error; the asm
-// takes the accumulator as an operand, so it is placed after the chain.
+// Extra wait states between the last MFMA of an accumulation chain and the first read of its
+// result.  On gfx950 the wait states the compiler inserts before reading a 16x16x4 MFMA result
+// were measured to be too few when the wave issues alone on its SIMD (the tail of a launch, or a
+// small system on an idle GPU): the read returned partly stale accumulator values -- wrong tips
+// in the last workgroup of an FP64 sweep, and rounding-level differences in FP32 between a
+// system solved alone and in a batch.  Sixteen more wait states remove both; the asm takes the
+// accumulator as an operand, so it is placed after the chain.
 template <typename T> __device__ __forceinline__ void tile_settle(Tile<T> &x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (sizeof(typename tr<T>::R) == 8) {
-        if constexpr (tr<T>::C) {
-            asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.re), "+a"(x.im));
-        } else {
-            asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.v));
-        }
+    if constexpr (tr<T>::C) {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.re), "+a"(x.im));
+    } else {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x.v));
     }
 #else
     (void)x;

@@ -14,6 +14,9 @@ Workloads (synthetic, seeded):
   c1           n = 10^4 pentadiagonal, a_ii = sum|a_ij| + 1, 4 partitions, 1 RHS
   c4           4096 independent systems in one batched solve: n log-uniform [10^3, 2*10^5],
                half-bandwidth uniform in {4, 8, 16, 32, 64}, 30% fill, dominance 1.1, nrhs 1..8
+  c5           request serving: Poisson arrivals (rate --rate), 80% real / 20% complex128, n
+               log-uniform [10^4, 5*10^6], half-bandwidth 2..128 (complex 2..64), 30% fill,
+               dominance 1.2, nrhs 1..64, 30% of requests repeat a resident matrix with a new RHS
 The CPU baseline is the oracle (oracle/, a C restatement of the reference's SPIKE flow with a
 banded partial-pivot LU in place of SuperLU_MT) timed on the host cores on the same matrix.
 
@@ -45,6 +48,29 @@ CONFIGS = {
 SEED_A, SEED_B = 20261202, 7
 C4 = dict(nsys=4096, n_lo=1e3, n_hi=2e5, ks=(4, 8, 16, 32, 64), fill=0.30, dom=1.1, nrhs_hi=8, tol=1e-10,
           seed=20261204)
+
+
+C5 = dict(nreq=120, rate=40.0, frac_complex=0.2, n_lo=1e4, n_hi=5e6, k_lo=2, k_hi=128, k_hi_complex=64,
+          nrhs_hi=64, repeat=0.3, fill=0.30, dom=1.2, tol=1e-10, seed=20261205)
+
+
+def c5_stream(nreq, seed):
+    """The C5 request stream: (arrival offset s, matrix id, nrhs, RHS seed) per request and
+    (dtype complex?, n, k, matrix seed) per distinct matrix, drawn from `seed`."""
+    rng = np.random.RandomState(seed)
+    t = np.cumsum(rng.exponential(1.0, nreq))
+    mats, reqs = [], []
+    for i in range(nreq):
+        if mats and rng.rand() < C5["repeat"]:
+            mid = int(rng.randint(len(mats)))
+        else:
+            cplx = rng.rand() < C5["frac_complex"]
+            n = int(np.exp(rng.uniform(np.log(C5["n_lo"]), np.log(C5["n_hi"]))))
+            k = int(rng.randint(C5["k_lo"], (C5["k_hi_complex"] if cplx else C5["k_hi"]) + 1))
+            mats.append((cplx, n, k, seed * 131 + len(mats)))
+            mid = len(mats) - 1
+        reqs.append((float(t[i]), mid, int(rng.randint(1, C5["nrhs_hi"] + 1)), seed * 977 + i))
+    return reqs, mats
 
 
 def c4_systems(nsys, seed):
@@ -92,8 +118,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--nsys", type=int, default=C4["nsys"], help="c4: number of systems in the batch")
+    ap.add_argument("--nreq", type=int, default=C5["nreq"], help="c5: number of requests")
+    ap.add_argument("--rate", type=float, default=C5["rate"], help="c5: Poisson arrival rate (requests/s)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison runs")
     ap.add_argument("--probe-copy", action="store_true",
@@ -118,6 +146,8 @@ def main():
 
     if args.config == "c4":
         return bench_batch(args, torch, hs)
+    if args.config == "c5":
+        return bench_serve(args, hs)
     cfg = CONFIGS[args.config]
     n, k, nrhs = cfg["n"], cfg["k"], cfg["nrhs"]
     t_gen = time.time()
@@ -254,6 +284,111 @@ def main():
         "stage_ms": {**{k: round(v, 4) for k, v in avg.items()}, **{k: round(v, 4) for k, v in avg_other.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_serve(args, hs):
+    """C5: Poisson request stream through the request server (spike_server_*); throughput is the
+    RHS solved over the span from the first arrival to the last completion."""
+    import threading
+    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)
+    reqs, mats = c5_stream(args.nreq, C5["seed"])
+    t_gen = time.time()
+    A = []
+    for j, (cplx, n, k, sd) in enumerate(mats):
+        A.append(hs.gen_banded(n, k, k, fill=C5["fill"], dom=(C5["dom"], C5["dom"]), diag_floor=1.0, seed=sd,
+                               complex=cplx))
+        if (j + 1) % 16 == 0:
+            log("c5: %d of %d matrices generated" % (j + 1, len(mats)))
+    Bs = [hs.gen_dense(A[mid].n, nrhs, seed=sb, complex=mats[mid][0]) for (_, mid, nrhs, sb) in reqs]
+    nnz = sum(a.nnz for a in A)
+    rhs = sum(r[2] for r in reqs)
+    log("generated c5: %d requests, %d distinct matrices (nnz %d), %d RHS in %.1f s" %
+        (len(reqs), len(mats), nnz, rhs, time.time() - t_gen))
+
+    def run(precision):
+        srv = hs.Server(precision, tol=C5["tol"], workers=3, max_batch=8,
+                        verbose=os.environ.get("HSPIKE_VERBOSE", "") == "1")
+        tickets = [None] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, (ta, mid, nrhs, _) in enumerate(reqs):
+                dt = t0 + ta / args.rate - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[i] = srv.submit(A[mid], Bs[i], key=mid + 1)     # the client names its matrices
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        infos = []
+        for i in range(len(reqs)):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            X, info = srv.wait(tickets[i])
+            infos.append(info)
+            if (i + 1) % 40 == 0:
+                log("c5 %s: %d of %d requests done" % (precision, i + 1, len(reqs)))
+        th.join()
+        span = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        bad = [x for x in infos if x["status"] != 0]
+        lat = np.array([x["latency_ms"] for x in infos])
+        return dict(span=span, stats=st, failed=len(bad), lat=lat, maxrel=max(x["max_rel_residual"] for x in infos),
+                    hits=sum(x["resident_hit"] for x in infos))
+
+    mixed = run("mixed")
+    log("mixed: span %.2f s, p50 %.1f ms, p99 %.1f ms, failed %d" % (mixed["span"], np.percentile(mixed["lat"], 50),
+                                                                      np.percentile(mixed["lat"], 99), mixed["failed"]))
+    f64 = None
+    if not args.no_fp64:
+        r = run("fp64")
+        f64 = {"rhs_per_s": round(rhs / r["span"], 2), "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+               "p99_ms": round(float(np.percentile(r["lat"], 99)), 3), "failed": r["failed"],
+               "max_rel_residual": r["maxrel"]}
+        log("fp64: %s" % f64)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+        threads = min(16, os.cpu_count() or 1)
+        t0, done, cnt = time.perf_counter(), 0, 0
+        for i, (_, mid, nrhs, _) in enumerate(reqs):
+            a = A[mid]
+            k = mats[mid][2]
+            if a.nnz > 20_000_000:          # keep the sample bounded: requests up to 2e7 nonzeros
+                continue
+            oracle.spike_solve(oracle.CSR(a.n, a.rowptr, a.colind, a.aij), Bs[i], max(1, min(4, a.n // (4 * k + 4))),
+                               nthreads=threads, want_reduced=False)
+            done += nrhs
+            cnt += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(done / t_cpu, 3), "unit": "RHS/s", "cores": threads, "kind": "port",
+               "sample": "%d of %d requests in arrival order, those up to 2e7 nonzeros (%d RHS), solved one after "
+                         "another by the oracle (FP64 SPIKE, up to 4 partitions), %.1f s" % (cnt, len(reqs), done, t_cpu)}
+    rhs_s = rhs / mixed["span"]
+    line = {
+        "metric": METRIC, "value": round(rhs_s, 2), "unit": "RHS/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
+        "ms_per_step": round(mixed["span"] * 1e3 / len(reqs), 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "C5: Poisson request stream, %.0f req/s, 80%% real / 20%% complex128, n log-uniform "
+                               "[1e4, 5e6], half-bandwidth 2-128 (complex 2-64), nrhs 1-64, 30%% repeat-RHS" % args.rate,
+                   "requests": len(reqs), "distinct_matrices": len(mats), "nnz_total": nnz, "nrhs_total": rhs,
+                   "arrival_rate_per_s": args.rate, "fill": C5["fill"], "seed": C5["seed"], "workers": 3, "max_batch": 8,
+                   "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},
+        "latency_ms": {"p50": round(float(np.percentile(mixed["lat"], 50)), 3),
+                       "p99": round(float(np.percentile(mixed["lat"], 99)), 3)},
+        "accuracy": {"max_rel_residual": mixed["maxrel"], "failed": mixed["failed"]},
+        "serving": {"resident_hits": mixed["hits"], "peak_device_bytes": mixed["stats"]["peak_bytes"],
+                    "budget_bytes": mixed["stats"]["budget_bytes"], "span_s": round(mixed["span"], 3)},
+        "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     return 0

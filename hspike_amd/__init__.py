@@ -8,6 +8,7 @@ ROCm device (only their data pointers are used).
   dspike_core_host(A, B) / zspike_core_host / sspike_core_host / cspike_core_host
   Solver            factor-once / solve-many handle (precision, partitions, tips, stats)
   Batch             many independent systems of uneven size in one set of launches
+  Server            request loop: continuous batching, memory admission, resident factors
   gen_banded        deterministic synthetic banded matrices (splitmix64 per row)
   rcm / permute     reverse Cuthill-McKee reordering of a general sparse matrix
 """
@@ -19,7 +20,7 @@ from . import _lib
 from ._lib import SPIKE_DTYPE_D, SPIKE_DTYPE_Z, SPIKE_PREC_FP64, SPIKE_PREC_MIXED, SpikeError, check, load
 
 __all__ = [
-    "CSRMatrix", "Solver", "Batch", "gen_banded", "gen_dense", "dspike_core_host", "dspike_core_host_blocking",
+    "CSRMatrix", "Solver", "Batch", "Server", "gen_banded", "gen_dense", "dspike_core_host", "dspike_core_host_blocking",
     "zspike_core_host", "sspike_core_host", "cspike_core_host", "residual", "rcm", "permute", "autotune_partitions",
     "SpikeError", "load",
 ]
@@ -235,6 +236,71 @@ class Batch:
         st = _lib.Stats()
         check(self._L.spike_batch_get_stats(self._h, ctypes.byref(st), _ptr(rel)), "spike_batch_get_stats")
         return rel, st.as_dict()
+
+
+class Server:
+    """Request server (include/spike_ext.h, spike_server_*): continuous batching of arriving solves,
+    admission by device memory, factors kept resident for repeated right-hand sides.
+
+        srv = Server("mixed", workers=3, max_batch=16)
+        t = srv.submit(A, B)          # returns a ticket; A, B stay referenced until wait()
+        X, info = srv.wait(t)         # info: status, latency_ms, max_rel_residual, resident_hit
+    """
+
+    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, workers=3, max_batch=16, mem_budget_gb=0.0,
+                 verbose=False):
+        self._L = load()
+        o = _lib.ServerOpts()
+        self._L.spike_server_default_opts(ctypes.byref(o))
+        o.opts = _lib.default_opts(precision=_prec_code(precision), tol=float(tol), max_refine=int(max_refine),
+                                   verbose=int(bool(verbose)))
+        o.workers, o.max_batch = int(workers), int(max_batch)
+        o.mem_budget_bytes = int(mem_budget_gb * 2**30)
+        h = ctypes.c_void_p()
+        check(self._L.spike_server_create(ctypes.byref(h), ctypes.byref(o)), "spike_server_create")
+        self._h = h
+        self._live = {}
+
+    def submit(self, A, B, key=0):
+        cplx = A.dtype_code == SPIKE_DTYPE_Z
+        dt = np.complex128 if cplx else np.float64
+        B = np.asfortranarray(np.asarray(B, dtype=dt).reshape(A.n, -1))
+        X = np.zeros_like(B, order="F")
+        r = _lib.Request(dtype=A.dtype_code, n=A.n, nnz=A.nnz, rowptr=A.rowptr.ctypes.data, colind=A.colind.ctypes.data,
+                         aij=A.aij.ctypes.data, nrhs=B.shape[1], b=B.ctypes.data, x=X.ctypes.data, key=int(key))
+        t = ctypes.c_uint64()
+        check(self._L.spike_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)), "spike_server_submit")
+        self._live[t.value] = (A, B, X)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        res = _lib.RequestResult()
+        rc = self._L.spike_server_wait(self._h, ctypes.c_uint64(ticket), float(timeout_ms), ctypes.byref(res))
+        if rc == 1:
+            return None, None
+        A, B, X = self._live.pop(ticket)
+        info = {"status": res.status, "latency_ms": res.latency_ms, "max_rel_residual": res.max_rel_residual,
+                "resident_hit": bool(res.resident_hit), "key": res.key}
+        return X, info
+
+    def stats(self):
+        st = _lib.ServerStats()
+        check(self._L.spike_server_get_stats(self._h, ctypes.byref(st)), "spike_server_get_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        check(self._L.spike_server_reset_stats(self._h), "spike_server_reset_stats")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.spike_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _legacy(fn_name, A, B, dt_np, cast_a=None):

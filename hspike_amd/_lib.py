@@ -65,6 +65,36 @@ class System(ctypes.Structure):
     ]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("opts", Opts), ("workers", ctypes.c_int), ("max_batch", ctypes.c_int),
+                ("mem_budget_bytes", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("dtype", ctypes.c_int), ("n", ctypes.c_int), ("nnz", ctypes.c_int64), ("rowptr", ctypes.c_void_p),
+        ("colind", ctypes.c_void_p), ("aij", ctypes.c_void_p), ("nrhs", ctypes.c_int), ("b", ctypes.c_void_p),
+        ("x", ctypes.c_void_p), ("key", ctypes.c_uint64),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("latency_ms", ctypes.c_double), ("max_rel_residual", ctypes.c_double),
+                ("resident_hit", ctypes.c_int), ("key", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_int64), ("failed", ctypes.c_int64), ("resident_hits", ctypes.c_int64),
+        ("rhs_solved", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double), ("resident_matrices", ctypes.c_int64),
+        ("resident_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every symbol declared in include/*.h (checked by the CPU test suite)
 EXPORTED = [
     # spike_interfaces.h
@@ -86,6 +116,8 @@ EXPORTED = [
     "spike_error_string", "spike_last_error_message", "spike_batch_create", "spike_batch_destroy", "spike_batch_solve_dev",
     "spike_batch_get_stats", "spike_autotune_partitions", "spike_residual_host", "spike_gen_banded",
     "spike_gen_dense", "spike_rcm_order", "spike_permute_csr",
+    "spike_server_default_opts", "spike_server_create", "spike_server_destroy", "spike_server_submit",
+    "spike_server_wait", "spike_server_get_stats", "spike_server_reset_stats",
 ]
 
 _LIB = None
@@ -155,6 +187,13 @@ def load():
         "matrix_SaveCSR": (i32, [ctypes.c_char_p, vp]),
         "matrix_ExtractMatrix": (vp, [vp, i32, i32, i32, i32, i32]),
         "matrix_AreEqual": (i32, [vp, vp]),
+        "spike_server_default_opts": (None, [ctypes.POINTER(ServerOpts)]),
+        "spike_server_create": (i32, [ctypes.POINTER(vp), ctypes.POINTER(ServerOpts)]),
+        "spike_server_destroy": (i32, [vp]),
+        "spike_server_submit": (i32, [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+        "spike_server_wait": (i32, [vp, ctypes.c_uint64, dbl, ctypes.POINTER(RequestResult)]),
+        "spike_server_get_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "spike_server_reset_stats": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)

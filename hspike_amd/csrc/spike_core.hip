@@ -20,6 +20,11 @@
 #include <chrono>
 #include <mutex>
 #include <memory>
+#include <thread>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <numeric>
 
 #include "spike_dev.h"
 #include "spike_kernels.inc"
@@ -274,7 +279,7 @@ void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartD
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
                   typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, int ymax,
                   const int *active) {
-    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(tr<T>::C && NT > 4))
+    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_sweep<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
                            vb, wt, wb, yt, yb, ymax, active);
 }
@@ -283,7 +288,7 @@ template <typename T, int NT, int CT>
 void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                     const T *U, const T *Bt, const T *Ct, const typename tr<T>::H *red, const int *part_iface, int ymax,
                     T *scratch, int accumulate, const int *active) {
-    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(tr<T>::C && NT > 4))
+    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, Bt, Ct, red,
                            part_iface, ymax, (Q4<T> *)scratch, accumulate, active);
 }
@@ -370,7 +375,7 @@ int Engine::factor() {
             S.ku = bw[3 * i];
             S.kl = bw[3 * i + 1];
             S.NT = block_nt(S.ku, S.kl);
-            if (S.NT == 0 || (dtype == SPIKE_DTYPE_Z && S.NT > 4)) {
+            if (S.NT == 0 || (dtype == SPIKE_DTYPE_Z && S.NT > 4 && o.precision == SPIKE_PREC_FP64)) {
                 set_err("bandwidth %ld exceeds this build's limit%s", "", (long)std::max(S.ku, S.kl));
                 return SPIKE_ERR_UNSUPPORTED;
             }
@@ -1270,3 +1275,5 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
 }
 
 }  // extern "C"
+
+#include "spike_server.inc"

@@ -149,6 +149,62 @@ int  spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int p
 int  spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,
                          const void *aij, int nrhs, const void *x, const void *b, double *rel);
 
+/* ---------------- request server: continuous batching, memory admission, resident factors --
+ * Requests (host arrays, kept alive and unmodified by the caller until spike_server_wait returns)
+ * are admitted as worker slots and device memory free up; new matrices of one dtype are factored
+ * and solved together (up to max_batch per job), jobs on different workers overlap on the GPU,
+ * and every factored matrix stays resident in HBM (least-recently-used eviction) keyed by a
+ * 64-bit fingerprint of its CSR or the caller's key, so a repeat request only runs the solve.
+ * A request's result does not depend on what else is in flight.  No counterpart in the
+ * reference (its driver solves one system per call). */
+typedef struct {
+    spike_opts_t opts;            /* precision, tolerance, refinement cap of every solve         */
+    int          workers;         /* concurrent jobs (one HIP stream each), default 3            */
+    int          max_batch;       /* new matrices per factor job, default 16                      */
+    uint64_t     mem_budget_bytes;/* device bytes for resident factors + jobs; 0 = 85% of free   */
+} spike_server_opts_t;
+
+typedef struct {
+    int          dtype;           /* SPIKE_DTYPE_D or SPIKE_DTYPE_Z                               */
+    int          n;
+    int64_t      nnz;
+    const int   *rowptr;          /* host CSR, int32, 0-based                                     */
+    const int   *colind;
+    const void  *aij;
+    int          nrhs;
+    const void  *b;               /* host n x nrhs column-major                                   */
+    void        *x;               /* host n x nrhs column-major, written on completion            */
+    uint64_t     key;             /* 0: fingerprint the CSR; otherwise the caller's matrix key    */
+} spike_request_t;
+
+typedef struct {
+    int      status;              /* SPIKE_SUCCESS, SPIKE_ERR_NOT_CONVERGED, or an error          */
+    double   latency_ms;          /* submit -> solution copied back                               */
+    double   max_rel_residual;
+    int      resident_hit;        /* 1: solved on resident factors                                */
+    uint64_t key;
+} spike_request_result_t;
+
+typedef struct {
+    int64_t  completed, failed, resident_hits, rhs_solved;
+    double   p50_ms, p99_ms, mean_ms, elapsed_ms;
+    int64_t  resident_matrices;
+    uint64_t resident_bytes, peak_bytes, budget_bytes;
+} spike_server_stats_t;
+
+typedef struct spike_server spike_server_t;
+
+void spike_server_default_opts(spike_server_opts_t *o);
+int  spike_server_create(spike_server_t **s, const spike_server_opts_t *o);
+/* Waits for every submitted request, then stops the workers. */
+int  spike_server_destroy(spike_server_t *s);
+int  spike_server_submit(spike_server_t *s, const spike_request_t *r, uint64_t *ticket);
+/* Blocks until the request is done (timeout_ms < 0: no limit; returns 1 on timeout, else the
+ * request's status); the ticket is released on return. */
+int  spike_server_wait(spike_server_t *s, uint64_t ticket, double timeout_ms, spike_request_result_t *res);
+int  spike_server_get_stats(spike_server_t *s, spike_server_stats_t *st);
+int  spike_server_reset_stats(spike_server_t *s);
+
 /* ---------------- synthetic generators (splitmix64 per row; deterministic) -------------- */
 /* Banded matrix: each in-band off-diagonal entry present with probability `fill`
  * (fill >= 1 means all), values U[-1,1] (complex: re and im U[-1,1]); diagonal =

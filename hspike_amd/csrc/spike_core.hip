@@ -25,6 +25,8 @@
 #include <deque>
 #include <map>
 #include <numeric>
+#include <tuple>
+#include <initializer_list>
 
 #include "spike_dev.h"
 #include "spike_kernels.inc"
@@ -82,6 +84,35 @@ struct DevBuf {
         return on;
     }
     template <typename X> X *as() const { return (X *)p; }
+};
+
+// Pinned host memory mapped into the device address space: the staging area of small transfers.
+struct PinBuf {
+    char *h = nullptr;
+    char *d = nullptr;
+    size_t bytes = 0;
+    PinBuf() = default;
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf() { if (h) (void)hipHostFree(h); }
+    int ensure(size_t need) {
+        if (bytes >= need) return 0;
+        if (h) (void)hipHostFree(h);
+        h = d = nullptr;
+        bytes = 0;
+        void *p = nullptr, *dp = nullptr;
+        if (hipHostMalloc(&p, need, hipHostMallocDefault) != hipSuccess ||
+            hipHostGetDevicePointer(&dp, p, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (p) (void)hipHostFree(p);
+            set_err("pinned host allocation of %s%ld bytes failed", "", (long)need);
+            return SPIKE_ERR_OOM;
+        }
+        h = (char *)p;
+        d = (char *)dp;
+        bytes = need;
+        return 0;
+    }
 };
 
 // Host -> device copy of a small descriptor array; completes before returning because the host
@@ -204,6 +235,71 @@ struct Engine {
     int ngroups = 0;
     bool factored = false;
     DevBuf d_bw, d_bwchunks, d_bwsys;
+    // small transfers staged through pinned host memory and moved by a copy kernel on the
+    // engine's stream: uploads take consecutive slices (no wait), a readback waits for the
+    // stream and then recycles the whole area
+    PinBuf pin;
+    size_t pin_off = 0;
+    static constexpr size_t PIN_BYTES = (size_t)1 << 20, PIN_MAX = (size_t)256 << 10;
+    int copy_words(const void *src, void *dst, size_t bytes) {
+        const int64_t nw = (int64_t)(bytes / 4);
+        if (nw <= 0) return 0;
+        const int grid = (int)std::min<int64_t>((nw + 255) / 256, 256);
+        hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(256), 0, stream, (const uint32_t *)src, (uint32_t *)dst, nw);
+        HIPCK(hipGetLastError());
+        return 0;
+    }
+    // device <- host, asynchronous; falls back to a DMA copy (synchronous) for large arrays
+    int put_small(DevBuf &d, const void *host, size_t bytes) {
+        int e;
+        if ((e = d.ensure(bytes))) return e;
+        if (!bytes) return 0;
+        if (bytes > PIN_MAX || bytes % 4) return upload(d, host, bytes, stream);
+        if ((e = pin.ensure(PIN_BYTES))) return e;
+        if (pin_off + bytes > pin.bytes) { HIPCK(hipStreamSynchronize(stream)); pin_off = 0; }
+        memcpy(pin.h + pin_off, host, bytes);
+        if ((e = copy_words(pin.d + pin_off, d.p, bytes))) return e;
+        pin_off += (bytes + 255) & ~(size_t)255;
+        return 0;
+    }
+    // host <- device for several arrays with one wait: list of (host dst, device src, bytes)
+    int get_small(std::initializer_list<std::tuple<void *, const void *, size_t>> parts) {
+        int e;
+        if ((e = pin.ensure(PIN_BYTES))) return e;
+        size_t need = 0;
+        bool dma = false;
+        for (auto &p : parts) {
+            need += (std::get<2>(p) + 255) & ~(size_t)255;
+            dma = dma || std::get<2>(p) % 4;
+        }
+        if (dma || need > PIN_MAX) {
+            for (auto &p : parts)
+                if (std::get<2>(p)) HIPCK(hipMemcpyAsync(std::get<0>(p), std::get<1>(p), std::get<2>(p), hipMemcpyDeviceToHost, stream));
+            HIPCK(hipStreamSynchronize(stream));
+            pin_off = 0;
+            return 0;
+        }
+        if (pin_off + need > pin.bytes) { HIPCK(hipStreamSynchronize(stream)); pin_off = 0; }
+        size_t off = pin_off;
+        for (auto &p : parts) {
+            if ((e = copy_words(std::get<1>(p), pin.d + off, std::get<2>(p)))) return e;
+            off += (std::get<2>(p) + 255) & ~(size_t)255;
+        }
+        HIPCK(hipStreamSynchronize(stream));
+        off = pin_off;
+        for (auto &p : parts) {
+            memcpy(std::get<0>(p), pin.h + off, std::get<2>(p));
+            off += (std::get<2>(p) + 255) & ~(size_t)255;
+        }
+        pin_off = 0;
+        return 0;
+    }
+    int zero_ints(DevBuf &d, size_t count) {
+        if (!count) return 0;
+        hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(256), 0, stream, d.as<int>(), (int64_t)count, 0);
+        HIPCK(hipGetLastError());
+        return 0;
+    }
     std::vector<uint64_t> bw_key;           // systems described by d_bwchunks / d_bwsys
     DevBuf own_rowptr, own_colind, own_aij;   // host-matrix copies (spike_set_matrix_host)
     DevBuf stage_b, stage_x;                   // host solve staging
@@ -295,15 +391,21 @@ void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const Par
 
 // FP64 residual with TW column lanes per row: the narrowest of 4, 8, 16 covering the chunk's
 // columns (all chunks share the partials stride `ymax`)
-int launch_residual(int dtype, int mcols, int ymax, unsigned nchunks, hipStream_t s, const RowChunk *chunks,
-                    const SysDev *sys, double *partials, const int *active) {
-    const int tw = mcols <= 4 ? 4 : (mcols <= 8 ? 8 : 16);
-#define RES_LAUNCH(H, TW) hipLaunchKernelGGL((k_residual<H, TW>), dim3(nchunks), dim3(256), 0, s, chunks, sys, partials, ymax, active)
+// (TW lanes per row, CPL columns per lane): pairs of columns per lane when the solver keeps the
+// row-major iterate (row stride even), and the fewest lanes per row that cover the columns
+int launch_residual(int dtype, int mcols, int ymax, bool rowmajor, unsigned nchunks, hipStream_t s,
+                    const RowChunk *chunks, const SysDev *sys, double *partials, const int *active) {
+    const int cpl = (rowmajor && dtype == SPIKE_DTYPE_D) ? 2 : 1;     // complex pairs cost too many registers
+    const int need = (mcols + cpl - 1) / cpl;
+    const int tw = need <= 4 ? 4 : (need <= 8 ? 8 : 16);
+#define RES_LAUNCH(H, TW, CPL) hipLaunchKernelGGL((k_residual<H, TW, CPL>), dim3(nchunks), dim3(256), 0, s, chunks, sys, partials, ymax, active)
+#define RES_TW(H, CPL) do { if (tw == 4) RES_LAUNCH(H, 4, CPL); else if (tw == 8) RES_LAUNCH(H, 8, CPL); else RES_LAUNCH(H, 16, CPL); } while (0)
     if (dtype == SPIKE_DTYPE_Z) {
-        if (tw == 4) RES_LAUNCH(zd, 4); else if (tw == 8) RES_LAUNCH(zd, 8); else RES_LAUNCH(zd, 16);
+        if (cpl == 2) RES_TW(zd, 2); else RES_TW(zd, 1);
     } else {
-        if (tw == 4) RES_LAUNCH(double, 4); else if (tw == 8) RES_LAUNCH(double, 8); else RES_LAUNCH(double, 16);
+        if (cpl == 2) RES_TW(double, 2); else RES_TW(double, 1);
     }
+#undef RES_TW
 #undef RES_LAUNCH
     HIPCK(hipGetLastError());
     return 0;
@@ -360,15 +462,13 @@ int Engine::factor() {
             if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
             bw_key = key;
         }
-        if ((e = d_bw.ensure(3 * sizeof(int) * nsys))) return e;
-        HIPCK(hipMemsetAsync(d_bw.p, 0, 3 * sizeof(int) * nsys, s));
+        if ((e = d_bw.ensure(3 * sizeof(int) * nsys)) || (e = zero_ints(d_bw, 3 * (size_t)nsys))) return e;
         if (!ch.empty())
             hipLaunchKernelGGL(k_bandwidth, dim3((unsigned)ch.size()), dim3(256), 0, s, d_bwchunks.as<RowChunk>(),
                                d_bwsys.as<SysDev>(), d_bw.as<int>());
         HIPCK(hipGetLastError());
         std::vector<int> bw(3 * nsys);
-        HIPCK(hipMemcpyAsync(bw.data(), d_bw.p, bw.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
+        if ((e = get_small({{bw.data(), d_bw.p, bw.size() * sizeof(int)}}))) return e;
         for (int i = 0; i < nsys; ++i) {
             if (bw[3 * i + 2]) { set_err("malformed CSR in system %s%ld", "", (long)i); return SPIKE_ERR_MALFORMED_CSR; }
             SysHost &S = sys[i];
@@ -485,8 +585,7 @@ int Engine::factor() {
             if ((e = b->ensure(np * bb * hs))) return e;
         if ((e = g.T.ensure(std::max<size_t>(nif, 1) * bb * hs))) return e;
         if ((e = g.pivinfo.ensure(np * 2 * sizeof(double)))) return e;
-        if ((e = g.status.ensure(4 * sizeof(int)))) return e;
-        HIPCK(hipMemsetAsync(g.status.p, 0, 4 * sizeof(int), s));
+        if ((e = g.status.ensure(4 * sizeof(int))) || (e = zero_ints(g.status, 4))) return e;
         // zero the scattered arrays (fill kernels; no memset nodes in code a caller may capture)
         T_SWITCH(tcode, {
             if ((e = fill_zero<T>(g.D, g.nblk * bb, s)) || (e = fill_zero<T>(g.U, g.nblk * bb, s)) ||
@@ -599,9 +698,8 @@ int Engine::factor() {
         Group &g = *groups[gi];
         int stv[4];
         g.piv_host.resize(g.parts.size() * 2);
-        HIPCK(hipMemcpyAsync(stv, g.status.p, sizeof stv, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(g.piv_host.data(), g.pivinfo.p, g.piv_host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
+        if ((e = get_small({{stv, g.status.p, sizeof stv}, {g.piv_host.data(), g.pivinfo.p, g.piv_host.size() * sizeof(double)}})))
+            return e;
         if (stv[0]) { set_err("matrix entry outside the detected band%s", ""); return SPIKE_ERR_MALFORMED_CSR; }
         for (size_t q = 0; q < g.parts.size(); ++q)
             if (g.piv_host[2 * q + 1] != 0.0) {
@@ -611,7 +709,7 @@ int Engine::factor() {
         if (stv[1] && !g.ifs.empty()) {
             // a 4x4 panel pivot of the MFMA inversion was zero: redo T_i with full partial pivoting
             const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
-            HIPCK(hipMemsetAsync(g.status.p, 0, 4 * sizeof(int), s));
+            if ((e = zero_ints(g.status, 4))) return e;
             if (dtype == SPIKE_DTYPE_Z) {
                 HIPCK(hipFuncSetAttribute((const void *)k_red_setup<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
                 hipLaunchKernelGGL(k_red_setup<zd>, dim3((unsigned)g.ifs.size()), dim3(256), smem, s, g.d_ifs.as<IfaceDesc>(),
@@ -624,8 +722,7 @@ int Engine::factor() {
                                    g.b, g.status.as<int>() + 1, (const int *)nullptr);
             }
             HIPCK(hipGetLastError());
-            HIPCK(hipMemcpyAsync(stv, g.status.p, sizeof stv, hipMemcpyDeviceToHost, s));
-            HIPCK(hipStreamSynchronize(s));
+            if ((e = get_small({{stv, g.status.p, sizeof stv}}))) return e;
         }
         if (stv[1]) { set_err("singular reduced-system block%s", ""); return SPIKE_ERR_SINGULAR; }
     }
@@ -664,7 +761,8 @@ int Engine::solve_chunk(int c0, int ymax) {
             return e;
         size_t rrows = 0;
         for (size_t gs = 0; gs < ngs; ++gs) rrows += sys[g.sys[gs]].n;
-        if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(rrows * ymax * hs))) return e;
+        const int ldrm = (ymax + 1) & ~1;      // even: a lane's column pair is one aligned load
+        if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(rrows * ldrm * hs))) return e;
         active[gi].assign(ngs, 0);
         std::vector<uint64_t> key = {(uint64_t)c0, (uint64_t)ymax, (uint64_t)(uintptr_t)g.rwork.p,
                                      (uint64_t)(uintptr_t)g.xrm.p};
@@ -684,14 +782,14 @@ int Engine::solve_chunk(int c0, int ymax) {
             sd.bptr = (const char *)S.d_b + (size_t)c0 * S.ldb * hs;
             sd.xptr = (char *)S.d_x + (size_t)c0 * S.ldx * hs;
             sd.rptr = (char *)g.rwork.p + roff * ymax * hs;
-            sd.xrm = (char *)g.xrm.p + roff * ymax * hs;
-            sd.ldrm = ymax;
+            sd.xrm = (char *)g.xrm.p + roff * ldrm * hs;
+            sd.ldrm = ldrm;
             roff += S.n;
             active[gi][gs] = m > 0;
             S.refine_iters = 0;
         }
         if (cached) {
-            if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+            if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
             continue;
         }
         if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
@@ -708,7 +806,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                 memset(&it, 0, sizeof it);
                 it.part = q; it.kind = ITEM_Y; it.col0 = c; it.ncols = std::min(cpi, sd.m - c);
                 it.dst = (char *)sd.xptr + (size_t)c * S.ldx * hs; it.lddst = S.ldx;
-                it.dst_rm = (void *)sd.xrm; it.ldrm = ymax;
+                it.dst_rm = (void *)sd.xrm; it.ldrm = sd.ldrm;
                 it.src = (const char *)sd.bptr + (size_t)c * S.ldb * hs; it.ldsrc = S.ldb;
                 if (d.P > 1) yb_.push_back(it);
                 rb_.push_back(it);
@@ -724,7 +822,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             (e = upload(g.items_rb, rb_.data(), rb_.size() * sizeof(SweepItem), s)) ||
             (e = upload(g.items_rr, rr_.data(), rr_.size() * sizeof(SweepItem), s)))
             return e;
-        if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+        if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
         if ((e = g.d_red_active.ensure(ngs * sizeof(int)))) return e;
         g.solve_key = key;
     }
@@ -759,7 +857,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             if (dtype == SPIKE_DTYPE_Z) { if ((e = fill_zero<zd>(g.red0, nif * slot_elems, s))) return e; }
             else if ((e = fill_zero<double>(g.red0, nif * slot_elems, s))) return e;
             std::vector<int> ract = active[gi];
-            if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
+            if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
             DevBuf *xo = &g.red0, *xn = &g.red1;
             std::vector<double> conv(nif * ncc * 2);
             std::vector<double> dmax(ngs), vmax(ngs);
@@ -791,8 +889,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                     std::swap(xo, xn);
                     ++iters;
                 }
-                HIPCK(hipMemcpyAsync(conv.data(), g.conv.p, conv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-                HIPCK(hipStreamSynchronize(s));
+                if ((e = get_small({{conv.data(), g.conv.p, conv.size() * sizeof(double)}}))) return e;
                 std::fill(dmax.begin(), dmax.end(), 0.0);
                 std::fill(vmax.begin(), vmax.end(), 0.0);
                 for (size_t q = 0; q < nif * ncc; ++q) {
@@ -812,7 +909,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                     if (!std::isfinite(rel) || rel > 1e3) { ract[gs] = 0; S.red_update = rel; }
                 }
                 if (!more) break;
-                if ((e = upload(g.d_red_active, ract.data(), ngs * sizeof(int), s))) return e;
+                if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
             }
             redcur[gi] = xo;     // last written iterate
             g.red_last = (xo == &g.red1) ? 1 : 0;
@@ -836,7 +933,7 @@ int Engine::solve_chunk(int c0, int ymax) {
         // ---- FP64 residual of every column
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
-            if ((e = launch_residual(dtype, ymax, ymax, (unsigned)g.chunks.size(), s, g.d_chunks.as<RowChunk>(),
+            if ((e = launch_residual(dtype, ymax, ymax, true, (unsigned)g.chunks.size(), s, g.d_chunks.as<RowChunk>(),
                                      g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.d_active.as<int>())))
                 return e;
             hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(g.sys.size() * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
@@ -849,8 +946,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             Group &g = *groups[gi];
             const size_t ngs = g.sys.size();
             std::vector<double> nr(ngs * ymax * 2);
-            HIPCK(hipMemcpyAsync(nr.data(), g.norms.p, nr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-            HIPCK(hipStreamSynchronize(s));
+            if ((e = get_small({{nr.data(), g.norms.p, nr.size() * sizeof(double)}}))) return e;
             bool changed = false;
             for (size_t gs = 0; gs < ngs; ++gs) {
                 if (!active[gi][gs]) continue;
@@ -867,7 +963,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                 else any = true;
             }
             if (changed && any)
-                if ((e = upload(g.d_active, active[gi].data(), ngs * sizeof(int), s))) return e;
+                if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
         }
         t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 12);
     }
@@ -1261,7 +1357,7 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
         if ((e = upload(dch, ch.data(), ch.size() * sizeof(RowChunk), 0)) || (e = upload(dsys, &sd, sizeof sd, 0)) ||
             (e = upload(dc0, c0s, sizeof c0s, 0)) || (e = dpart.ensure((size_t)nch * 64 * 2 * 8)) || (e = dnorm.ensure(64 * 2 * 8)))
             return e;
-        if ((e = launch_residual(dtype, m, 64, (unsigned)nch, 0, dch.as<RowChunk>(), dsys.as<SysDev>(), dpart.as<double>(),
+        if ((e = launch_residual(dtype, m, 64, false, (unsigned)nch, 0, dch.as<RowChunk>(), dsys.as<SysDev>(), dpart.as<double>(),
                                  (const int *)nullptr)))
             return e;
         hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),

@@ -212,6 +212,7 @@ struct Group {
     int64_t nblk = 0;
     int64_t rows_pad = 0;                   // nblk * b
     DevBuf d_parts, d_ifs, d_part_iface, d_chunks, d_chunk0, d_sysdev, d_active, d_red_active;
+    DevBuf blkmap;                          // partition of each block
     DevBuf D, U, C, R, Bt, Ct;              // factor type
     DevBuf vt, vb, wt, wb, T;               // H type
     DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
@@ -574,6 +575,9 @@ int Engine::factor() {
             if ((e = upload(g.d_chunks, g.chunks.data(), g.chunks.size() * sizeof(RowChunk), s))) return e;
             if ((e = upload(g.d_chunk0, g.chunk0.data(), g.chunk0.size() * sizeof(int), s))) return e;
             if ((e = upload(g.d_sysdev, g.sysdev.data(), g.sysdev.size() * sizeof(SysDev), s))) return e;
+            if ((e = g.blkmap.ensure(g.nblk * sizeof(int)))) return e;
+            hipLaunchKernelGGL(k_blkmap, dim3((unsigned)np), dim3(256), 0, s, g.d_parts.as<PartDesc>(), g.blkmap.as<int>());
+            HIPCK(hipGetLastError());
             g.vw_uploaded = false;
         }
         DevBuf *tb[] = {&g.D, &g.U, &g.C, &g.R};
@@ -586,12 +590,10 @@ int Engine::factor() {
         if ((e = g.T.ensure(std::max<size_t>(nif, 1) * bb * hs))) return e;
         if ((e = g.pivinfo.ensure(np * 2 * sizeof(double)))) return e;
         if ((e = g.status.ensure(4 * sizeof(int))) || (e = zero_ints(g.status, 4))) return e;
-        // zero the scattered arrays (fill kernels; no memset nodes in code a caller may capture)
+        // zero the coupling tiles, which the conversion scatters into (fill kernels; no memset
+        // nodes in code a caller may capture); D, U and C are written whole by the conversion
         T_SWITCH(tcode, {
-            if ((e = fill_zero<T>(g.D, g.nblk * bb, s)) || (e = fill_zero<T>(g.U, g.nblk * bb, s)) ||
-                (e = fill_zero<T>(g.C, g.nblk * bb, s)) || (e = fill_zero<T>(g.Bt, np * bb, s)) ||
-                (e = fill_zero<T>(g.Ct, np * bb, s)))
-                return e;
+            if ((e = fill_zero<T>(g.Bt, np * bb, s)) || (e = fill_zero<T>(g.Ct, np * bb, s))) return e;
         });
         g.plan_uploaded = true;
     }
@@ -600,10 +602,10 @@ int Engine::factor() {
         Group &g = *groups[gi];
         const int np = (int)g.parts.size();
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
-            hipLaunchKernelGGL((k_csr_to_blocks<T, NT>), dim3((unsigned)g.chunks.size()), dim3(256), 0, s,
-                               g.d_chunks.as<RowChunk>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(),
+            hipLaunchKernelGGL((k_csr_to_blocks<T, NT>), dim3((unsigned)(g.nblk * NT)), dim3(256), 0, s,
+                               g.blkmap.as<int>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(),
                                g.D.as<T>(), g.U.as<T>(), g.C.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(),
-                               g.status.as<int>(), (const int *)nullptr);
+                               g.status.as<int>());
         }));
         HIPCK(hipGetLastError());
         (void)np;

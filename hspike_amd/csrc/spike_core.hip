@@ -155,14 +155,60 @@ double now_ms() {
 
 }  // namespace
 
+#define NT_SWITCH(NTV, BODY)                                 \
+    switch (NTV) {                                           \
+    case 1: { constexpr int NT = 1; BODY; } break;           \
+    case 2: { constexpr int NT = 2; BODY; } break;           \
+    case 4: { constexpr int NT = 4; BODY; } break;           \
+    case 8: { constexpr int NT = 8; BODY; } break;           \
+    default: break;                                          \
+    }
+
+#define T_SWITCH(TC, BODY)                                   \
+    switch (TC) {                                            \
+    case 0: { typedef float T; BODY; } break;                \
+    case 1: { typedef double T; BODY; } break;               \
+    case 2: { typedef zf T; BODY; } break;                   \
+    case 3: { typedef zd T; BODY; } break;                   \
+    default: break;                                          \
+    }
+
 // ============================================================================ autotuner
-// Partition count from (n, k, nrhs, dtype, precision) only, so a system's schedule never
-// depends on the batch it is solved in.  Each partition is one workgroup chain of nb blocks;
-// about 24 blocks per partition keep the spike couplings small enough for the reduced
-// block-Jacobi iteration to converge in a few sweeps while giving a 10^6-row, k=64 system
-// ~650 independent chains (2-3 per CU on 256 CUs).
+namespace {
+
+// Factor workgroups the device runs at once for block size 16*NT and factor type tcode: CUs x
+// workgroups per CU of k_factor (HIP occupancy query; 0 when no device is visible).  Cached.
+int factor_slots(int tcode, int nt) {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const int key = tcode * 16 + nt;
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int dev = 0, cus = 0, per = 0;
+    const void *fn = nullptr;
+    T_SWITCH(tcode, NT_SWITCH(nt, { fn = (const void *)(k_factor<T, NT>); }));
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        !fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt * 64, 0) != hipSuccess)
+        cus = per = 0;
+    (void)hipGetLastError();
+    const int slots = cus * per;
+    if (getenv("HSPIKE_VERBOSE"))
+        fprintf(stderr, "[hspike] factor slots tcode=%d NT=%d: %d CUs x %d workgroups\n", tcode, nt, cus, per);
+    cache[key] = slots;
+    return slots;
+}
+
+}  // namespace
+
+// Partition count from (n, k, nrhs, dtype, precision) and the device only, so a system's
+// schedule never depends on the batch it is solved in.  Each partition is one workgroup chain of
+// nb blocks.  About 12 blocks per partition keep the spike couplings small enough for the reduced
+// block-Jacobi iteration to converge in a few sweeps; a large system is capped at one round of
+// the factor kernel (the workgroups the device runs at once): past that, more partitions only
+// add rounds of the same chains plus a larger reduced system.
 extern "C" int spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int precision) {
-    (void)nrhs; (void)dtype; (void)precision;
+    (void)nrhs;
     const int NT = block_nt(ku, kl);
     if (NT == 0 || n <= 0) return 1;
     const int b = 16 * NT;
@@ -173,6 +219,8 @@ extern "C" int spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dt
     }();
     const long rows_target = blocks_per_part * b;
     long P = n / rows_target;
+    const long slots = factor_slots(type_code(dtype, precision), NT);
+    if (slots > 0) P = std::min(P, slots);
     const long min_rows = std::max(2 * b, ku + kl);
     P = std::min(P, (long)n / std::max(1L, min_rows));
     P = std::max(1L, std::min(P, 8192L));
@@ -342,24 +390,6 @@ struct Engine {
 };
 
 // ---------------------------------------------------------------------------- dispatch
-#define NT_SWITCH(NTV, BODY)                                 \
-    switch (NTV) {                                           \
-    case 1: { constexpr int NT = 1; BODY; } break;           \
-    case 2: { constexpr int NT = 2; BODY; } break;           \
-    case 4: { constexpr int NT = 4; BODY; } break;           \
-    case 8: { constexpr int NT = 8; BODY; } break;           \
-    default: break;                                          \
-    }
-
-#define T_SWITCH(TC, BODY)                                   \
-    switch (TC) {                                            \
-    case 0: { typedef float T; BODY; } break;                \
-    case 1: { typedef double T; BODY; } break;               \
-    case 2: { typedef zf T; BODY; } break;                   \
-    case 3: { typedef zd T; BODY; } break;                   \
-    default: break;                                          \
-    }
-
 // Column tiles per sweep workgroup: CT * NT * sizeof(T) <= 64 keeps the two LDS exchange panels
 // (2 * 16*CT * b elements) within 32 KB and the per-lane registers (CT accumulator tiles per
 // product, NT-deep operand slices) low enough for two or more waves per SIMD.
@@ -879,12 +909,20 @@ int Engine::solve_chunk(int c0, int ymax) {
                         });
                     } else {
                         NT_SWITCH(g.NT, {
-                            hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
-                                               g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
-                                               (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
-                                               (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
-                                               (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
-                                               ymax, g.d_red_active.as<int>());
+                            if constexpr (NT <= 4)
+                                hipLaunchKernelGGL((k_red_jacobi_q<double, NT>), dim3((unsigned)(nif * ncc)), dim3(NT * 64), 0, s,
+                                                   g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
+                                                   (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
+                                                   (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
+                                                   (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
+                                                   ymax, g.d_red_active.as<int>());
+                            else
+                                hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
+                                                   g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
+                                                   (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
+                                                   (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
+                                                   (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
+                                                   ymax, g.d_red_active.as<int>());
                         });
                     }
                     HIPCK(hipGetLastError());

@@ -262,6 +262,7 @@ struct Group {
     DevBuf d_parts, d_ifs, d_part_iface, d_chunks, d_chunk0, d_sysdev, d_active, d_red_active;
     DevBuf blkmap;                          // partition of each block
     DevBuf D, U, C, R, Bt, Ct;              // factor type
+    DevBuf Wf, Vf;                          // forward-swept W spike (all blocks) and V spike (last two blocks)
     DevBuf vt, vb, wt, wb, T;               // H type
     DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
     DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
@@ -404,20 +405,20 @@ int ct_for(int ncols, int ctmax) { return std::min(ctmax, ncols <= 16 ? 1 : (nco
 template <typename T, int NT, int CT>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
-                  typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, int ymax,
-                  const int *active) {
+                  typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, T *Wf, T *Vf,
+                  T *wsave, int ymax, const int *active) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_sweep<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
-                           vb, wt, wb, yt, yb, ymax, active);
+                           vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active);
 }
 
 template <typename T, int NT, int CT>
-void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
-                    const T *U, const T *Bt, const T *Ct, const typename tr<T>::H *red, const int *part_iface, int ymax,
-                    T *scratch, int accumulate, const int *active) {
+void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *U, const T *Wf,
+                    const T *Vf, const typename tr<T>::H *red, const int *part_iface, int ymax, const T *wsave,
+                    int accumulate, const int *active) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
-        hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, Bt, Ct, red,
-                           part_iface, ymax, (Q4<T> *)scratch, accumulate, active);
+        hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, U, Wf, Vf, red,
+                           part_iface, ymax, (const Q4<T> *)wsave, accumulate, active);
 }
 
 // FP64 residual with TW column lanes per row: the narrowest of 4, 8, 16 covering the chunk's
@@ -610,9 +611,10 @@ int Engine::factor() {
             HIPCK(hipGetLastError());
             g.vw_uploaded = false;
         }
-        DevBuf *tb[] = {&g.D, &g.U, &g.C, &g.R};
+        DevBuf *tb[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf};
         for (auto b : tb)
             if ((e = b->ensure(g.nblk * bb * ts))) return e;
+        if ((e = g.Vf.ensure(np * 2 * bb * ts))) return e;
         if ((e = g.Bt.ensure(np * bb * ts)) || (e = g.Ct.ensure(np * bb * ts))) return e;
         DevBuf *hb[] = {&g.vt, &g.vb, &g.wt, &g.wb};
         for (auto b : hb)
@@ -671,7 +673,8 @@ int Engine::factor() {
                     (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                             g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                             g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
-                                            (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+                                            (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
+                                            (const int *)nullptr);
                 })));
                 HIPCK(hipGetLastError());
             }
@@ -700,7 +703,8 @@ int Engine::factor() {
             (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                     g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                     g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
-                                    (H *)nullptr, (H *)nullptr, 0, (const int *)nullptr);
+                                    (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
+                                    (const int *)nullptr);
         })));
         HIPCK(hipGetLastError());
     }
@@ -840,10 +844,10 @@ int Engine::solve_chunk(int c0, int ymax) {
                 it.dst = (char *)sd.xptr + (size_t)c * S.ldx * hs; it.lddst = S.ldx;
                 it.dst_rm = (void *)sd.xrm; it.ldrm = sd.ldrm;
                 it.src = (const char *)sd.bptr + (size_t)c * S.ldb * hs; it.ldsrc = S.ldb;
-                if (d.P > 1) yb_.push_back(it);
+                yb_.push_back(it);        // every partition: the recovery reads its forward sweep
                 rb_.push_back(it);
                 it.src = (const char *)sd.rptr + (size_t)c * S.n * hs; it.ldsrc = S.n;
-                if (d.P > 1) yr_.push_back(it);
+                yr_.push_back(it);
                 rr_.push_back(it);
             }
         }
@@ -872,8 +876,8 @@ int Engine::solve_chunk(int c0, int ymax) {
                 typedef typename tr<T>::H H;
                 (launch_sweep<T, NT, CT>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                         g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
-                                        (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(), g.ymax,
-                                        g.d_active.as<int>());
+                                        (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(),
+                                        (T *)nullptr, (T *)nullptr, g.scratch.as<T>(), g.ymax, g.d_active.as<int>());
             })));
             HIPCK(hipGetLastError());
         }
@@ -962,10 +966,9 @@ int Engine::solve_chunk(int c0, int ymax) {
             const DevBuf &items = it == 0 ? g.items_rb : g.items_rr;
             T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
-                (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
-                                          g.C.as<T>(), g.U.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (const H *)redcur[gi]->p,
-                                          g.d_part_iface.as<int>(), g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0,
-                                          g.d_active.as<int>());
+                (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.U.as<T>(),
+                                          g.Wf.as<T>(), g.Vf.as<T>(), (const H *)redcur[gi]->p, g.d_part_iface.as<int>(),
+                                          g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0, g.d_active.as<int>());
             })));
             HIPCK(hipGetLastError());
         }

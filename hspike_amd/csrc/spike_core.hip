@@ -648,7 +648,8 @@ int Engine::factor() {
         const int np = (int)g.parts.size();
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
-                               g.U.as<T>(), (const T *)g.C.as<T>(), g.pivinfo.as<double>(), (const int *)nullptr);
+                               g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), g.pivinfo.as<double>(),
+                               (const int *)nullptr);
         }));
         HIPCK(hipGetLastError());
     }
@@ -657,8 +658,9 @@ int Engine::factor() {
         Group &g = *groups[gi];
         const int np = (int)g.parts.size();
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
-            hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
-                               (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr);
+            if constexpr (!factor_forms_r<T, NT>())        // else the factor kernel formed R
+                hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
+                                   (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr);
         }));
         HIPCK(hipGetLastError());
     }

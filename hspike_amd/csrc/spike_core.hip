@@ -402,13 +402,13 @@ int ct_max(int tcode, int NT) {
 }
 int ct_for(int ncols, int ctmax) { return std::min(ctmax, ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 4)); }
 
-template <typename T, int NT, int CT>
+template <typename T, int NT, int CT, bool VW>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
                   typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, T *Wf, T *Vf,
                   T *wsave, int ymax, const int *active) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
-        hipLaunchKernelGGL((k_sweep<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
+        hipLaunchKernelGGL((k_sweep<T, NT, CT, VW>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
                            vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active);
 }
 
@@ -672,7 +672,7 @@ int Engine::factor() {
             if (g.n_items_vw) {
                 T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
                     typedef typename tr<T>::H H;
-                    (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
+                    (launch_sweep<T, NT, CT, true>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                             g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                             g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                             (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
@@ -702,7 +702,7 @@ int Engine::factor() {
         if ((e = upload(g.items_vw, items.data(), items.size() * sizeof(SweepItem), s))) return e;
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
             typedef typename tr<T>::H H;
-            (launch_sweep<T, NT, CT>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
+            (launch_sweep<T, NT, CT, true>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                     g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                     g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                     (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
@@ -876,7 +876,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             const DevBuf &items = it == 0 ? g.items_yb : g.items_yr;
             T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
-                (launch_sweep<T, NT, CT>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
+                (launch_sweep<T, NT, CT, false>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                         g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
                                         (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(),
                                         (T *)nullptr, (T *)nullptr, g.scratch.as<T>(), g.ymax, g.d_active.as<int>());

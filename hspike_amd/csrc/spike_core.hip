@@ -11,6 +11,7 @@
 // The reference's flow is OpenMP/spike_interfaces.c:43-419; partition rows follow
 // OpenMP/spike_analysis.c:38-76.
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,8 +39,11 @@ namespace {
 
 thread_local char g_errmsg[512];
 
-void set_err(const char *fmt, const char *a = "", long v = 0) {
-    snprintf(g_errmsg, sizeof g_errmsg, fmt, a, v);
+__attribute__((format(printf, 1, 2))) void set_err(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_errmsg, sizeof g_errmsg, fmt, ap);
+    va_end(ap);
 }
 
 #define HIPCK(call)                                                                  \
@@ -243,6 +247,7 @@ struct SysHost {
     void *d_x = nullptr;
     int ldx = 0;
     double maxrel = 0.0;
+    double maxrel_before = 0.0;             // largest final residual of the earlier RHS chunks
     int refine_iters = 0, red_iters = 0;
     double red_update = 0.0;
 };
@@ -508,7 +513,7 @@ int Engine::factor() {
             S.kl = bw[3 * i + 1];
             S.NT = block_nt(S.ku, S.kl);
             if (S.NT == 0 || (dtype == SPIKE_DTYPE_Z && S.NT > 4 && o.precision == SPIKE_PREC_FP64)) {
-                set_err("bandwidth %ld exceeds this build's limit%s", "", (long)std::max(S.ku, S.kl));
+                set_err("bandwidth %ld exceeds this build's limit", (long)std::max(S.ku, S.kl));
                 return SPIKE_ERR_UNSUPPORTED;
             }
             S.P = o.partitions > 0 ? o.partitions
@@ -516,7 +521,7 @@ int Engine::factor() {
             if (S.P > S.n) S.P = std::max(1, S.n);
             const int nreg = S.n / S.P;
             if (S.P > 1 && (nreg < S.ku + S.kl || nreg < std::max(S.ku, S.kl) || nreg < 1)) {
-                set_err("partitions too small for the bandwidth (rows per partition %ld)%s", "", (long)nreg);
+                set_err("partitions too small for the bandwidth (rows per partition %ld)", (long)nreg);
                 return SPIKE_ERR_INVALID_ARG;
             }
         }
@@ -825,6 +830,7 @@ int Engine::solve_chunk(int c0, int ymax) {
             roff += S.n;
             active[gi][gs] = m > 0;
             S.refine_iters = 0;
+            S.maxrel_before = c0 == 0 ? 0.0 : S.maxrel;
         }
         if (cached) {
             if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
@@ -1003,7 +1009,7 @@ int Engine::solve_chunk(int c0, int ymax) {
                     mr = std::isfinite(rel) ? std::max(mr, rel) : INFINITY;
                 }
                 S.refine_iters = it;
-                if (c0 == 0) S.maxrel = mr; else S.maxrel = std::max(S.maxrel, mr);
+                S.maxrel = std::max(S.maxrel_before, mr);     // this chunk's latest pass only
                 if (mr <= o.tol || it == max_refine || !std::isfinite(mr)) { active[gi][gs] = 0; changed = true; }
                 else any = true;
             }

@@ -339,6 +339,11 @@ def bench_serve(args, hs):
         st = srv.stats()
         srv.close()
         bad = [x for x in infos if x["status"] != 0]
+        if bad:
+            codes = {}
+            for x in bad:
+                codes[x["status"]] = codes.get(x["status"], 0) + 1
+            log("c5 %s: failed requests by status %s" % (precision, codes))
         lat = np.array([x["latency_ms"] for x in infos])
         return dict(span=span, stats=st, failed=len(bad), lat=lat, maxrel=max(x["max_rel_residual"] for x in infos),
                     hits=sum(x["resident_hit"] for x in infos))

@@ -1047,6 +1047,15 @@ int Engine::solve(int) {
     return SPIKE_SUCCESS;
 }
 
+// per-stage times of `src` added to (or copied into) `dst`
+void stage_times(spike_stats_t &dst, const spike_stats_t &src, bool add) {
+    double spike_stats_t::*f[] = {&spike_stats_t::ms_convert, &spike_stats_t::ms_factor, &spike_stats_t::ms_rtop,
+                                  &spike_stats_t::ms_spikes, &spike_stats_t::ms_red_setup, &spike_stats_t::ms_ysweep,
+                                  &spike_stats_t::ms_reduced, &spike_stats_t::ms_recover, &spike_stats_t::ms_residual,
+                                  &spike_stats_t::ms_factor_total, &spike_stats_t::ms_solve_total};
+    for (auto m : f) dst.*m = add ? dst.*m + src.*m : src.*m;
+}
+
 void fill_stats(const Engine &E, spike_stats_t *s) {
     *s = E.st;
     if (!E.sys.empty()) {
@@ -1316,6 +1325,13 @@ int spike_get_reduced_solution(const spike_handle_t *h, double *xr) {
 // ============================================================================ batched systems
 struct spike_batch {
     Engine E;
+    // A batch whose arrays do not fit in HBM is solved in windows of at most `cap` systems, one
+    // after another (each system's result is the one it gets alone, so windows change no bits);
+    // `cap` halves on every failed allocation and is kept for later calls.
+    int cap = INT32_MAX;
+    int windows = 0;
+    std::vector<double> rel;        // final residual of every system of the last call
+    spike_stats_t acc{};            // stage times summed over the windows of the last call
 };
 
 extern "C" {
@@ -1327,6 +1343,8 @@ int spike_batch_create(spike_batch_t **bt, const spike_opts_t *o) {
     spike_batch *x = new (std::nothrow) spike_batch();
     if (!x) return SPIKE_ERR_OOM;
     if (o) x->E.o = *o;
+    if (const char *v = getenv("HSPIKE_BATCH_WINDOW"))      // systems per window from the start (tests, tuning)
+        if (atoi(v) > 0) x->cap = atoi(v);
     *bt = x;
     return SPIKE_SUCCESS;
 }
@@ -1340,36 +1358,63 @@ int spike_batch_destroy(spike_batch_t *bt) {
 
 int spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_system_t *sys) {
     if (!bt || nsys <= 0 || !sys || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z)) return SPIKE_ERR_INVALID_ARG;
-    Engine &E = bt->E;
-    E.dtype = dtype;
-    E.sys.assign(nsys, SysHost());
     for (int i = 0; i < nsys; ++i) {
         const spike_system_t &q = sys[i];
         if (q.n <= 0 || q.nnz < 0 || q.nnz > INT32_MAX || !q.d_rowptr || q.nrhs < 0 || (q.nrhs > 0 && (!q.d_b || !q.d_x)))
             return SPIKE_ERR_INVALID_ARG;
-        SysHost &S = E.sys[i];
-        S.n = q.n; S.nnz = q.nnz; S.d_rowptr = q.d_rowptr; S.d_colind = q.d_colind; S.d_aij = q.d_aij;
-        S.nrhs = q.nrhs; S.d_b = q.d_b; S.ldb = q.n; S.d_x = q.d_x; S.ldx = q.n;
     }
-    E.factored = false;
+    Engine &E = bt->E;
+    bt->rel.assign(nsys, 0.0);
+    bt->acc = spike_stats_t{};
+    bt->windows = 0;
     const double t0 = now_ms();
-    int e = E.factor();
-    if (e) return e;
-    e = E.solve(0);
+    int e = SPIKE_SUCCESS;
+    for (int lo = 0; lo < nsys;) {
+        const int hi = (int)std::min<int64_t>(nsys, (int64_t)lo + bt->cap);
+        E.dtype = dtype;
+        E.sys.assign(hi - lo, SysHost());
+        for (int i = lo; i < hi; ++i) {
+            const spike_system_t &q = sys[i];
+            SysHost &S = E.sys[i - lo];
+            S.n = q.n; S.nnz = q.nnz; S.d_rowptr = q.d_rowptr; S.d_colind = q.d_colind; S.d_aij = q.d_aij;
+            S.nrhs = q.nrhs; S.d_b = q.d_b; S.ldb = q.n; S.d_x = q.d_x; S.ldx = q.n;
+        }
+        E.factored = false;
+        e = E.factor();
+        if (!e) e = E.solve(0);
+        if (e == SPIKE_ERR_OOM && hi - lo > 1) {      // free this window's arrays, retry it halved
+            (void)hipStreamSynchronize(E.stream);
+            E.groups.clear();
+            E.ngroups = 0;
+            bt->cap = (hi - lo) / 2;
+            continue;
+        }
+        for (int i = lo; i < hi; ++i) bt->rel[i] = E.sys[i - lo].maxrel;
+        stage_times(bt->acc, E.st, true);
+        ++bt->windows;
+        if (e) break;
+        lo = hi;
+    }
     if (E.o.verbose) {
         double mr = 0;
-        for (auto &S : E.sys) mr = std::max(mr, S.maxrel);
-        fprintf(stderr, "[spike batch] %d systems, %d groups, factor %.3f ms, solve %.3f ms, total %.3f ms, max rel %.3e\n",
-                nsys, E.ngroups, E.st.ms_factor_total, E.st.ms_solve_total, now_ms() - t0, mr);
+        for (double r : bt->rel) mr = std::max(mr, r);
+        fprintf(stderr, "[spike batch] %d systems in %d window(s), factor %.3f ms, solve %.3f ms, total %.3f ms, max rel %.3e\n",
+                nsys, bt->windows, bt->acc.ms_factor_total, bt->acc.ms_solve_total, now_ms() - t0, mr);
     }
     return e;
 }
 
 int spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s, double *rel) {
     if (!bt) return SPIKE_ERR_INVALID_ARG;
-    if (s) fill_stats(bt->E, s);
+    if (s) {
+        fill_stats(bt->E, s);
+        stage_times(*s, bt->acc, false);
+        double mr = 0;
+        for (double r : bt->rel) mr = std::max(mr, r);
+        s->max_rel_residual = mr;
+    }
     if (rel)
-        for (size_t i = 0; i < bt->E.sys.size(); ++i) rel[i] = bt->E.sys[i].maxrel;
+        for (size_t i = 0; i < bt->rel.size(); ++i) rel[i] = bt->rel[i];
     return SPIKE_SUCCESS;
 }
 

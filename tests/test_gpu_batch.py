@@ -77,3 +77,25 @@ def test_batch_repeat_is_deterministic(env):
     for a, s in zip(first, systems):
         assert torch.equal(a, s["x"])
     bt.close()
+
+
+def test_windowed_batch_matches_whole(env, monkeypatch):
+    """A batch too large for HBM is solved in windows of systems (the window halves on every
+    failed allocation); HSPIKE_BATCH_WINDOW starts it small.  Windows change no bits."""
+    torch, hs = env
+    systems = make_batch(torch, hs, 11, seed=21)
+    whole = hs.Batch("mixed", tol=1e-10)
+    rel_whole, st_whole = whole.solve_dev(systems)
+    ref = [s["x"].clone() for s in systems]
+    whole.close()
+    for s in systems:
+        s["x"].zero_()
+    monkeypatch.setenv("HSPIKE_BATCH_WINDOW", "3")
+    win = hs.Batch("mixed", tol=1e-10)
+    rel_win, st_win = win.solve_dev(systems)
+    torch.cuda.synchronize()
+    for a, s in zip(ref, systems):
+        assert torch.equal(a, s["x"])
+    np.testing.assert_array_equal(rel_whole, rel_win)
+    assert st_win["max_rel_residual"] <= 1e-10
+    win.close()

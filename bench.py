@@ -243,8 +243,8 @@ def main():
     # residual: a lane holds a pair of RHS columns; team width covers ceil(ymax / 2) lanes
     res_need = (min(nrhs, 64) + 1) // 2
     res_tw = 4 if res_need <= 4 else (8 if res_need <= 8 else 16)
-    kname = {"factor": "k_factor<float,%d>" % nt, "spikes": "k_sweep<float,%d,%d>" % (nt, ct_vw),
-             "recover": "k_recover<float,%d,%d>" % (nt, ct_y), "ysweep": "k_sweep<float,%d,%d>" % (nt, ct_y),
+    kname = {"factor": "k_factor<float,%d>" % nt, "spikes": "k_sweep<float,%d,%d,true>" % (nt, ct_vw),
+             "recover": "k_recover<float,%d,%d>" % (nt, ct_y), "ysweep": "k_sweep<float,%d,%d,false>" % (nt, ct_y),
              "residual": "k_residual<double,%d,2>" % res_tw}[dom]
     roof.update({"traffic": measured_traffic(args.config, kname), "kernel": kname,
                  "stage": dom, "algorithmic_flops": F, "algorithmic_bytes": Bytes, "ms_per_launch": round(per_launch_ms, 4)})

@@ -3,11 +3,13 @@
 // Pipeline per batch of systems (a single-system handle is a batch of one):
 //   validate + bandwidth (one launch over every system's rows)
 //   group systems by block size b = 16 * NT, NT in {1, 2, 4, 8}
-//   per group: CSR -> blocks, block LU (k_factor), top block rows (k_rtop), V/W spike tips
-//              (k_sweep), reduced-system setup (k_red_setup)                    -- spike_factor
-//   per RHS chunk: y tips (k_sweep), block-Jacobi reduced solve in FP64 (k_red_jacobi),
-//              recovery into the FP64 solution (k_recover), FP64 residual (k_residual);
-//              mixed precision repeats on the residual until every column meets tol.  -- solve
+//   per group: CSR -> blocks, block LU with the top block rows R_j (k_factor; k_rtop where the
+//              R panels do not fit in LDS), V/W spike tips keeping their forward sweeps
+//              (k_sweep), reduced-system setup (k_red_setup_mfma)                 -- spike_factor
+//   per RHS chunk: y tips keeping the forward sweep (k_sweep), block-Jacobi reduced solve in
+//              FP64 (k_red_jacobi_q / k_red_jacobi), recovery as one backward chain into the FP64
+//              solution (k_recover), FP64 residual (k_residual); mixed precision repeats on the
+//              residual until every column meets tol.                                -- solve
 // The reference's flow is OpenMP/spike_interfaces.c:43-419; partition rows follow
 // OpenMP/spike_analysis.c:38-76.
 #include <hip/hip_runtime.h>

@@ -58,7 +58,8 @@ typedef struct {
     /* per-kernel times from HIP events on the handle's stream (milliseconds) */
     double  ms_convert;        /* CSR -> blocks                                               */
     double  ms_factor;         /* block LU (k_factor)                                         */
-    double  ms_rtop;           /* top block rows of the U inverse (k_rtop)                    */
+    double  ms_rtop;           /* top block rows of the U inverse when a separate launch forms
+                                  them (k_rtop; otherwise the factor kernel does, inside ms_factor) */
     double  ms_spikes;         /* V / W spike tips (k_sweep)                                  */
     double  ms_red_setup;      /* reduced-system blocks (k_red_setup)                         */
     double  ms_ysweep;         /* RHS tips, summed over refinement passes (k_sweep)          */

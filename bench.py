@@ -221,6 +221,21 @@ def main():
     avg_other = {k: float(np.mean([st[k] for st in mixed["stats"]])) for k in
                  ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_factor_total", "ms_solve_total")}
     passes = max(1, mixed["last"]["refine_iters"] + 1)
+    ridge = PEAK_TFLOPS["f32"] * 1e12 / (HBM_PEAK_GBS * 1e9)
+    # every stage against both peaks; the binding one is the stage's arithmetic intensity side
+    # of the FP32 ridge point
+    stage_roof = {}
+    for st_name, st_ms in avg.items():
+        nl = passes if st_name in ("recover", "ysweep", "residual") else 1
+        if st_ms <= 0:
+            continue
+        f_st, b_st = algorithmic_counts(st_name, n, A.nnz, k, k, nrhs, P_auto, 4)
+        t_st = st_ms / nl * 1e-3
+        pk = PEAK_TFLOPS["f64"] if st_name == "residual" else PEAK_TFLOPS["f32"]
+        stage_roof[st_name] = {"launches": nl, "ms_per_launch": round(st_ms / nl, 4),
+                               "GB_s": round(b_st / t_st / 1e9, 2), "hbm_frac": round(b_st / t_st / 1e9 / HBM_PEAK_GBS, 4),
+                               "TFLOP_s": round(f_st / t_st / 1e12, 3), "mfma_frac": round(f_st / t_st / 1e12 / pk, 4),
+                               "bound": "mfma" if f_st / b_st >= pk * 1e12 / (HBM_PEAK_GBS * 1e9) else "hbm"}
     dom = max(avg, key=avg.get)
     launches = passes if dom in ("recover", "ysweep", "residual") else 1
     F, Bytes = algorithmic_counts(dom, n, A.nnz, k, k, nrhs, P_auto, 4)
@@ -228,7 +243,6 @@ def main():
     flop_rate = F / (per_launch_ms * 1e-3) / 1e12
     byte_rate = Bytes / (per_launch_ms * 1e-3) / 1e9
     inten = F / Bytes
-    ridge = PEAK_TFLOPS["f32"] * 1e12 / (HBM_PEAK_GBS * 1e9)
     if inten >= ridge:
         roof = {"bound": "mfma", "achieved": round(flop_rate, 3), "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
                 "frac": round(flop_rate / PEAK_TFLOPS["f32"], 4)}
@@ -286,6 +300,7 @@ def main():
         "resident_factors_rhs_per_s": round(nrhs / (mixed["resident_ms"] * 1e-3), 2),
         "stage_ms": {**{k: round(v, 4) for k, v in avg.items()}, **{k: round(v, 4) for k, v in avg_other.items()}},
         "roofline": roof,
+        "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

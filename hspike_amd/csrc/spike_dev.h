@@ -69,6 +69,15 @@ __device__ __forceinline__ double s_mag(double a) { return a < 0 ? -a : a; }
 template <typename R> __device__ __forceinline__ double s_mag(cx<R> a) {
     return (a.re < 0 ? -(double)a.re : (double)a.re) + (a.im < 0 ? -(double)a.im : (double)a.im);
 }
+// the same modulus in the component type (pivot search without conversions)
+__device__ __forceinline__ float  s_magr(float a)  { return __builtin_fabsf(a); }
+__device__ __forceinline__ double s_magr(double a) { return __builtin_fabs(a); }
+template <typename R> __device__ __forceinline__ R s_magr(cx<R> a) { return s_magr(a.re) + s_magr(a.im); }
+// reciprocal of a pivot: FP32 takes the hardware reciprocal (1 ulp; FP32 factors only seed the
+// refinement), FP64 and complex the correctly rounded division
+__device__ __forceinline__ float  s_recip_piv(float a)  { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double s_recip_piv(double a) { return 1.0 / a; }
+template <typename R> __device__ __forceinline__ cx<R> s_recip_piv(cx<R> a) { return s_recip(a); }
 __device__ __forceinline__ bool s_finite(float a)  { return __builtin_isfinite(a); }
 __device__ __forceinline__ bool s_finite(double a) { return __builtin_isfinite(a); }
 template <typename R> __device__ __forceinline__ bool s_finite(cx<R> a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }

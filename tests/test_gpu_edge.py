@@ -89,6 +89,34 @@ def test_reference_fixture_matrices(hs, oracle_mod, rel):
     assert hs.residual(A, X, B).max() <= 1e-10
 
 
+def pair_swap_matrix(hs, n, k, complex_=False, seed=0):
+    """Zero diagonal, 2 at (i, i xor 1), band noise with row sums <= 0.8: nonsingular (singular
+    values >= 1.2) but every 4x4 pivot block of the block inversion needs row interchanges."""
+    rng = np.random.default_rng(seed)
+    i = np.repeat(np.arange(n), 2 * k + 1)
+    j = i + np.tile(np.arange(-k, k + 1), n)
+    keep = (j >= 0) & (j < n)
+    i, j = i[keep], j[keep]
+    v = 0.8 / (2 * k) * rng.uniform(-1, 1, i.size)
+    if complex_:
+        v = v * np.exp(2j * np.pi * rng.uniform(0, 1, i.size))
+    v[j == i] = 0.0
+    v[j == (i ^ 1)] = 2.0
+    rp = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rp, i + 1, 1)
+    return hs.CSRMatrix(n, np.cumsum(rp).astype(np.int32), j.astype(np.int32), v)
+
+
+@pytest.mark.parametrize("k,prec,cplx", [(8, "fp64", False), (8, "mixed", False), (40, "mixed", False),
+                                         (40, "fp64", False), (8, "mixed", True), (24, "fp64", True)])
+def test_interchanges_inside_pivot_blocks(hs, oracle_mod, k, prec, cplx):
+    """A zero diagonal: the 4x4 pivot blocks are inverted with row interchanges (without them the
+    factorization stops at a zero pivot)."""
+    A = pair_swap_matrix(hs, 3000, k, cplx, seed=k)
+    B = hs.gen_dense(A.n, 3, seed=9, complex=cplx)
+    solve_and_check(hs, oracle_mod, A, B, prec, P=3)
+
+
 def test_singular_pivot_reported(hs):
     n = 200
     rp = np.arange(n + 1, dtype=np.int32)

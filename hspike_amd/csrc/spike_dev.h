@@ -162,6 +162,28 @@ template <typename T> __device__ __forceinline__ void tile_settle(Tile<T> &x) {
 #endif
 }
 
+// The same wait states once after the chains of N independent accumulator tiles: one asm that
+// takes every tile keeps all of their MFMAs ahead of it (settling tile by tile lets the
+// scheduler put sixteen wait states between consecutive independent MFMAs).
+template <typename T, int N> __device__ __forceinline__ void tiles_settle(Tile<T> (&x)[N]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (!tr<T>::C && N == 2) {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x[0].v), "+a"(x[1].v));
+    } else if constexpr (!tr<T>::C && N == 3) {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x[0].v), "+a"(x[1].v), "+a"(x[2].v));
+    } else if constexpr (!tr<T>::C && N == 4) {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x[0].v), "+a"(x[1].v), "+a"(x[2].v), "+a"(x[3].v));
+    } else if constexpr (tr<T>::C && N == 2) {
+        asm volatile("s_nop 7\n\ts_nop 7" : "+a"(x[0].re), "+a"(x[0].im), "+a"(x[1].re), "+a"(x[1].im));
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) tile_settle(x[i]);
+    }
+#else
+    (void)x;
+#endif
+}
+
 template <typename T> __device__ __forceinline__ Q4<T> tile_q(const Tile<T> &x) {
     Q4<T> r;
 #pragma unroll

@@ -54,7 +54,7 @@ class CSRMatrix:
 
     def bandwidth(self):
         ku, kl = ctypes.c_int(), ctypes.c_int()
-        check(load().matrix_ComputeBandwidth(self.n, _ptr(self.colind), _ptr(self.rowptr), ctypes.byref(ku),
+        check(load().matrix_ComputeBandwidth(self.n, _ptr(self.colind), _ptr(self.rowptr), None, ctypes.byref(ku),
                                              ctypes.byref(kl)), "matrix_ComputeBandwidth")
         return ku.value, kl.value
 
@@ -105,15 +105,28 @@ def _prec_code(precision):
     raise ValueError("precision must be 'fp64' or 'mixed'")
 
 
+SPIKE_RED_AUTO, SPIKE_RED_JACOBI, SPIKE_RED_DIRECT = 0, 1, 2
+
+
+def _red_code(solver):
+    codes = {"auto": SPIKE_RED_AUTO, "jacobi": SPIKE_RED_JACOBI, "direct": SPIKE_RED_DIRECT}
+    if solver in codes.values():
+        return int(solver)
+    if solver not in codes:
+        raise ValueError("reduced_solver must be 'auto', 'jacobi' or 'direct'")
+    return codes[solver]
+
+
 class Solver:
     """Factor-once / solve-many GPU SPIKE handle (include/spike_ext.h)."""
 
     def __init__(self, precision="fp64", partitions=0, tol=1e-10, max_refine=8, reduced_maxit=64,
-                 reduced_tol=1e-15, verbose=False):
+                 reduced_tol=1e-15, verbose=False, reduced_solver="auto"):
         self._L = load()
         self._opts = _lib.default_opts(precision=_prec_code(precision), partitions=int(partitions), tol=float(tol),
                                        max_refine=int(max_refine), reduced_maxit=int(reduced_maxit),
-                                       reduced_tol=float(reduced_tol), verbose=int(bool(verbose)))
+                                       reduced_tol=float(reduced_tol), verbose=int(bool(verbose)),
+                                       reduced_solver=_red_code(reduced_solver))
         h = ctypes.c_void_p()
         check(self._L.spike_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_create")
         self._h = h
@@ -177,22 +190,36 @@ class Solver:
         check(self._L.spike_get_schedule(self._h, ctypes.byref(P), _ptr(rows), P.value + 1), "spike_get_schedule")
         return rows
 
-    def tips(self, nrhs):
-        """FP64 spike tips in the oracle's layout: dict name -> [P, rows, cols]."""
+    def _tip_cols(self, nrhs):
+        # the library writes the RHS tips of the last solve's first column chunk (<= 64 columns)
+        m = min(self.stats()["nrhs"], 64)
+        if nrhs is not None and nrhs > m:
+            raise ValueError("the last solve had %d right-hand-side columns" % m)
+        return m
+
+    def tips(self, nrhs=None):
+        """FP64 spike tips in the oracle's layout: dict name -> [P, rows, cols] (RHS tips: the first
+        nrhs columns of the last solve, all of them by default)."""
+        m = self._tip_cols(nrhs)
         st = self.stats()
         P, ku, kl = st["P"], st["ku"], st["kl"]
-        shapes = {"Vt": (ku, ku), "Vb": (kl, ku), "Wt": (ku, kl), "Wb": (kl, kl), "yt": (ku, nrhs), "yb": (kl, nrhs)}
+        shapes = {"Vt": (ku, ku), "Vb": (kl, ku), "Wt": (ku, kl), "Wb": (kl, kl), "yt": (ku, m), "yb": (kl, m)}
         bufs = {k: np.zeros(P * r * c + 1) for k, (r, c) in shapes.items()}
         check(self._L.spike_get_tips(self._h, *[_ptr(bufs[k]) for k in ("Vt", "Vb", "Wt", "Wb", "yt", "yb")]),
               "spike_get_tips")
-        return {k: bufs[k][:P * r * c].reshape(P, c, r).transpose(0, 2, 1) for k, (r, c) in shapes.items()}
+        out = {k: bufs[k][:P * r * c].reshape(P, c, r).transpose(0, 2, 1) for k, (r, c) in shapes.items()}
+        if nrhs is not None:
+            out["yt"], out["yb"] = out["yt"][:, :, :nrhs], out["yb"][:, :, :nrhs]
+        return out
 
-    def reduced_solution(self, nrhs):
+    def reduced_solution(self, nrhs=None):
+        """Reduced solution xr (reference order, N_r x columns) of the last solve's first columns."""
+        m = self._tip_cols(nrhs)
         st = self.stats()
         Nr = st["P"] * (st["ku"] + st["kl"])
-        xr = np.zeros(Nr * nrhs)
+        xr = np.zeros(Nr * m)
         check(self._L.spike_get_reduced_solution(self._h, _ptr(xr)), "spike_get_reduced_solution")
-        return xr.reshape(nrhs, Nr).T
+        return xr.reshape(m, Nr).T[:, :(m if nrhs is None else nrhs)]
 
 
 class Batch:
@@ -349,6 +376,25 @@ def residual(A, X, B):
                                      _ptr(A.rowptr), _ptr(A.colind), _ptr(aij), X.shape[1], _ptr(X), _ptr(B),
                                      _ptr(rel)), "spike_residual_host")
     return rel
+
+
+def band_solve(A, B):
+    """X = A^{-1} B by the GPU band LU with partial pivoting and FP64 refinement
+    (spike_band_solve_host); returns (X, per-column relative residuals)."""
+    dt = np.complex128 if (A.dtype_code == SPIKE_DTYPE_Z or np.iscomplexobj(B)) else np.float64
+    B = np.asfortranarray(np.asarray(B, dtype=dt).reshape(A.n, -1))
+    X = np.zeros_like(B, order="F")
+    aij = np.ascontiguousarray(A.aij, dtype=dt)
+    rel = np.zeros(B.shape[1])
+    check(load().spike_band_solve_host(SPIKE_DTYPE_Z if dt == np.complex128 else SPIKE_DTYPE_D, A.n, A.nnz,
+                                       _ptr(A.rowptr), _ptr(A.colind), _ptr(aij), B.shape[1], _ptr(B), A.n, _ptr(X),
+                                       A.n, _ptr(rel)), "spike_band_solve_host")
+    return X, rel
+
+
+def host_solve(A, B):
+    """The reduced-system solver slot of the reference (directSolver_Host_Solve, spike_algebra.h:157-164)."""
+    return _legacy("directSolver_Host_Solve", A, B, np.float64)
 
 
 def rcm(A):

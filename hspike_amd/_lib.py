@@ -35,7 +35,7 @@ class Opts(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int), ("partitions", ctypes.c_int), ("max_refine", ctypes.c_int),
         ("tol", ctypes.c_double), ("reduced_maxit", ctypes.c_int), ("reduced_tol", ctypes.c_double),
-        ("sweep_waves", ctypes.c_int), ("verbose", ctypes.c_int),
+        ("sweep_waves", ctypes.c_int), ("verbose", ctypes.c_int), ("reduced_solver", ctypes.c_int),
     ]
 
 
@@ -50,7 +50,7 @@ class Stats(ctypes.Structure):
         ("ms_red_setup", ctypes.c_double), ("ms_ysweep", ctypes.c_double), ("ms_reduced", ctypes.c_double),
         ("ms_recover", ctypes.c_double), ("ms_residual", ctypes.c_double),
         ("ms_factor_total", ctypes.c_double), ("ms_solve_total", ctypes.c_double),
-        ("device_bytes", ctypes.c_size_t),
+        ("device_bytes", ctypes.c_size_t), ("reduced_direct", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -103,8 +103,11 @@ EXPORTED = [
     "matrix_LoadCSR", "matrix_SaveCSR", "matrix_LoadCSR_z", "matrix_LoadCSR_zl", "matrix_Deallocate", "matrix_z_Deallocate",
     "matrix_CreateEmptyMatrix", "matrix_CreateFromComponents", "matrix_AreEqual", "matrix_CheckCSR",
     "matrix_ComputeBandwidth", "matrix_ExtractMatrix", "matrix_ExtractBlock", "block_CreateEmptyBlock",
-    "block_Deallocate", "spike_solve_analysis_p", "schedule_Destroy", "matrix_CreateEmptyReducedSystem",
+    "block_Deallocate", "spike_solve_analysis", "spike_solve_analysis_p", "schedule_Destroy",
+    "matrix_CreateEmptyReducedSystem",
     "matrix_AssembleReducedSystem", "block_AssembleReducedRHS",
+    # spike_memory.h
+    "_spike_malloc", "spike_free", "spike_nullify", "spike_memory_is_pinned",
     # spike_hip.h
     "directSolver_CreateHandler", "directSolver_Configure", "directSolver_Factorize", "directSolver_SolveForRHS",
     "directSolver_ShowStatistics", "directSolver_Finalize", "directSolver_Solve", "directSolver_Host_Solve",
@@ -114,7 +117,8 @@ EXPORTED = [
     "spike_set_matrix_host", "spike_factor", "spike_solve_dev", "spike_solve_host", "spike_get_stats",
     "spike_get_schedule", "spike_get_tips", "spike_get_reduced_solution", "spike_set_stream",
     "spike_error_string", "spike_last_error_message", "spike_batch_create", "spike_batch_destroy", "spike_batch_solve_dev",
-    "spike_batch_get_stats", "spike_autotune_partitions", "spike_residual_host", "spike_gen_banded",
+    "spike_batch_get_stats", "spike_autotune_partitions", "spike_residual_host", "spike_band_solve_host",
+    "spike_gen_banded",
     "spike_gen_dense", "spike_rcm_order", "spike_permute_csr",
     "spike_server_default_opts", "spike_server_create", "spike_server_destroy", "spike_server_submit",
     "spike_server_wait", "spike_server_get_stats", "spike_server_reset_stats",
@@ -155,6 +159,7 @@ def load():
         "spike_batch_get_stats": (i32, [vp, ctypes.POINTER(Stats), vp]),
         "spike_autotune_partitions": (i32, [i32, i32, i32, i32, i32, i32]),
         "spike_residual_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, vp, vp]),
+        "spike_band_solve_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, vp]),
         "spike_gen_banded": (i32, [i32, i32, i32, i32, dbl, dbl, dbl, dbl, dbl, ctypes.c_uint64, vp, vp, vp,
                                    ctypes.POINTER(i64)]),
         "spike_gen_dense": (i32, [i32, i64, dbl, dbl, ctypes.c_uint64, vp]),
@@ -175,8 +180,15 @@ def load():
         "directSolver_Host_Solve": (i32, [i32, i32, i32, vp, vp, vp, vp, vp]),
         "ComputeResidualOfLinearSystem": (i32, [vp, vp, vp, vp, vp, i32, i32]),
         "matrix_CheckCSR": (i32, [i32, ctypes.c_uint64, vp, vp]),
-        "matrix_ComputeBandwidth": (i32, [i32, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
-        "matrix_CreateEmptyReducedSystem": (vp, [i32, vp, vp]),
+        "matrix_ComputeBandwidth": (i32, [i32, vp, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "matrix_CreateEmptyReducedSystem": (vp, [i32, vp, vp, vp]),
+        "spike_solve_analysis": (vp, [vp, i32]),
+        "spike_solve_analysis_p": (vp, [vp, i32, i32]),
+        "schedule_Destroy": (None, [vp]),
+        "_spike_malloc": (vp, [i32, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_char_p, i32]),
+        "spike_free": (None, [vp]),
+        "spike_nullify": (None, [vp]),
+        "spike_memory_is_pinned": (i32, [vp]),
         "matrix_AssembleReducedSystem": (i32, [vp, i32, i32, i32, vp, vp, vp, vp]),
         "block_AssembleReducedRHS": (i32, [vp, i32, i32, i32, i32, vp, vp]),
         "matrix_Deallocate": (i32, [vp]),

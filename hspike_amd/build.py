@@ -16,8 +16,8 @@ INC = os.path.join(ROOT, "include")
 OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libhspike.so")
 
-C_SOURCES = ["spike_matrix.c", "spike_interfaces.c", "spike_backend.c", "spike_gen.c", "spike_rcm.c"]
-HIP_SOURCES = ["spike_core.hip", "spike_server.hip"]
+C_SOURCES = ["spike_matrix.c", "spike_interfaces.c", "spike_backend.c", "spike_gen.c", "spike_rcm.c", "spike_memory.c"]
+HIP_SOURCES = ["spike_core.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HSPIKE_ARCH", "gfx950")
@@ -27,7 +27,10 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-fopenmp", "-Wall", "-Wno-unused-function", "-I" + INC]
+# the C sources call the HIP runtime API only (pinned host memory), hence the platform define
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-fopenmp", "-Wall", "-Wno-unused-function", "-I" + INC,
+           "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
 
 
 def _run(cmd):
@@ -50,8 +53,6 @@ def build(verbose=False, jobs=None):
         cmds.append(["gcc"] + C_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
     for src in HIP_SOURCES:
-        if not os.path.exists(os.path.join(CSRC, src)):
-            continue
         obj = os.path.join(bdir, src + ".o")
         cmds.append([HIPCC] + HIP_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)

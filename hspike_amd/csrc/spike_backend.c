@@ -108,10 +108,14 @@ Error_t directSolver_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t
     return e;
 }
 
+/* The reference solves the reduced system R with PARDISO (spike_algebra.c:263-420); R is banded
+ * (bandwidths ku + 2 kl - 1 and 2 ku + kl - 1 in the reference's ordering, wider than the SPIKE
+ * kernels' blocks), so it goes to the device band LU with partial pivoting and FP64 refinement. */
 Error_t directSolver_Host_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t *colind, integer_t *rowptr,
                                 double *aij, double *xij, double *bij)
 {
-    return directSolver_Solve(n, nnz, nrhs, colind, rowptr, aij, xij, bij);
+    if (n <= 0 || nnz < 0 || nrhs < 0) return SPIKE_ERR_INVALID_ARG;
+    return spike_band_solve_host(SPIKE_DTYPE_D, n, nnz, rowptr, colind, aij, nrhs, bij, n, xij, n, NULL);
 }
 
 Error_t ComputeResidualOfLinearSystem(integer_t *colind, integer_t *rowptr, double *aij, double *x, double *b,

@@ -34,6 +34,7 @@
 #include "spike_dev.h"
 #include "spike_kernels.inc"
 #include "spike_kernels_aux.inc"
+#include "spike_band.inc"
 #include "spike_ext.h"
 
 // ============================================================================ utilities
@@ -252,6 +253,7 @@ struct SysHost {
     double maxrel_before = 0.0;             // largest final residual of the earlier RHS chunks
     int refine_iters = 0, red_iters = 0;
     double red_update = 0.0;
+    bool red_direct = false;                // reduced system solved by the band LU (until the next factor)
 };
 
 struct Group {
@@ -281,6 +283,13 @@ struct Group {
     bool vw_uploaded = false;               // items_vw holds this plan's V/W items
     int red_last = 0;                       // buffer (0: red0, 1: red1) holding the last iterate
     std::vector<double> piv_host;
+    // direct solve of the reduced system: R of every system in rd_sys assembled in band storage
+    // and factored (banded LU with partial pivoting); the factors stay valid until the next factor
+    std::vector<int> rd_sys;                // group-local systems, in the order of rd_bs
+    bool rd_factored = false;
+    std::vector<BandSys> rd_bs;
+    std::vector<RedDirect> rd_desc_h;
+    DevBuf rd_desc, rd_pass, rd_bsd, rd_ab, rd_piv, rd_x, rd_info, rd_items;
 };
 
 struct Engine {
@@ -386,13 +395,15 @@ struct Engine {
         size_t s = 0;
         for (int gi = 0; gi < ngroups; ++gi) {
             const Group &g = *groups[gi];
-            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
-                                  &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork};
+            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb,
+                                  &g.T, &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork, &g.xrm,
+                                  &g.rd_ab, &g.rd_x};
             for (auto b : bs) s += b->bytes;
         }
         return s;
     }
     int factor();
+    int reduced_direct(Group &g, const std::vector<int> &which, DevBuf &xo, int ymax);
     int solve_chunk(int c0, int m);
     int solve(int nrhs_hint);
 };
@@ -479,6 +490,9 @@ int Engine::factor() {
     const double t0 = now_ms();
     hipStream_t s = stream;
     const int nsys = (int)sys.size();
+    // a new factorization invalidates the direct solves of the reduced systems
+    for (auto &S : sys) S.red_direct = false;
+    for (auto &gp : groups) { gp->rd_factored = false; gp->rd_sys.clear(); }
     HIPCK(hipEventRecord(ev[0], s));
 
     // ---- validation and bandwidth of every system (one launch)
@@ -521,8 +535,11 @@ int Engine::factor() {
             S.P = o.partitions > 0 ? o.partitions
                                    : spike_autotune_partitions(S.n, S.ku, S.kl, S.nrhs, dtype, o.precision);
             if (S.P > S.n) S.P = std::max(1, S.n);
+            // a partition needs at least max(ku, kl) rows, so that its coupling blocks B_i, C_i
+            // fit inside it; its top and bottom tips may overlap (short partitions of the
+            // reference's fixtures at P = 4), which leaves R nonsingular
             const int nreg = S.n / S.P;
-            if (S.P > 1 && (nreg < S.ku + S.kl || nreg < std::max(S.ku, S.kl) || nreg < 1)) {
+            if (S.P > 1 && (nreg < std::max(S.ku, S.kl) || nreg < 1)) {
                 set_err("partitions too small for the bandwidth (rows per partition %ld)", (long)nreg);
                 return SPIKE_ERR_INVALID_ARG;
             }
@@ -781,6 +798,140 @@ int Engine::factor() {
     return SPIKE_SUCCESS;
 }
 
+// ============================================================================ direct reduced solve
+// R of the group-local systems `which` (assembled and factored once per factorization; the set
+// only grows between factorizations), its right-hand side from the RHS tips of this pass, the
+// band solve, and the solution scattered into the interface iterate `xo` the recovery reads.
+int Engine::reduced_direct(Group &g, const std::vector<int> &which, DevBuf &xo, int ymax) {
+    hipStream_t s = stream;
+    int e;
+    const size_t hs = h_bytes(dtype);
+    bool grow = false;
+    for (int gs : which)
+        if (std::find(g.rd_sys.begin(), g.rd_sys.end(), gs) == g.rd_sys.end()) { g.rd_sys.push_back(gs); grow = true; }
+    const int nd = (int)g.rd_sys.size();
+    if (grow || !g.rd_factored) {
+        g.rd_bs.assign(nd, BandSys{});
+        g.rd_desc_h.assign(nd, RedDirect{});
+        int64_t ab = 0, pv = 0;
+        int maxP = 1;
+        size_t lds = 0;
+        for (int q = 0; q < nd; ++q) {
+            const SysHost &S = sys[g.sys[g.rd_sys[q]]];
+            int klr, kur;
+            red_band_shape(S.ku, S.kl, klr, kur);
+            BandSys &B = g.rd_bs[q];
+            B.n = S.P * (S.ku + S.kl);
+            B.kl = klr; B.ku = kur; B.ldab = 2 * klr + kur + 1;
+            B.ab = ab; B.piv = pv;
+            ab += (int64_t)B.n * B.ldab;
+            pv += B.n;
+            RedDirect &D = g.rd_desc_h[q];
+            D.part0 = g.sys_part0[g.rd_sys[q]];
+            D.iface0 = g.part_iface[D.part0];
+            D.P = S.P; D.ku = S.ku; D.kl = S.kl; D.band = q;
+            maxP = std::max(maxP, S.P);
+            lds = std::max(lds, (size_t)(std::max(klr, 1) + klr + kur + 1) * hs);
+        }
+        if (lds > 64 * 1024) {
+            set_err("reduced system bandwidth %ld too wide for the direct solver", (long)(lds / hs));
+            return SPIKE_ERR_UNSUPPORTED;
+        }
+        if ((e = g.rd_ab.ensure((size_t)ab * hs)) || (e = g.rd_piv.ensure((size_t)pv * sizeof(int))) ||
+            (e = g.rd_info.ensure(nd * sizeof(int))))
+            return e;
+        if (dtype == SPIKE_DTYPE_Z) { if ((e = fill_zero<zd>(g.rd_ab, (size_t)ab, s))) return e; }
+        else if ((e = fill_zero<double>(g.rd_ab, (size_t)ab, s))) return e;
+        if ((e = upload(g.rd_bsd, g.rd_bs.data(), nd * sizeof(BandSys), s)) ||
+            (e = upload(g.rd_desc, g.rd_desc_h.data(), nd * sizeof(RedDirect), s)))
+            return e;
+        if (dtype == SPIKE_DTYPE_Z) {
+            hipLaunchKernelGGL(k_red_band<zd>, dim3(maxP, nd), dim3(256), 0, s, g.rd_desc.as<RedDirect>(), g.rd_bsd.as<BandSys>(),
+                               (const zd *)g.vt.p, (const zd *)g.vb.p, (const zd *)g.wt.p, (const zd *)g.wb.p, g.b,
+                               g.rd_ab.as<zd>());
+            hipLaunchKernelGGL(k_gbtrf<zd>, dim3(nd), dim3(GB_THREADS), lds, s, g.rd_bsd.as<BandSys>(), g.rd_ab.as<zd>(),
+                               g.rd_piv.as<int>(), g.rd_info.as<int>());
+        } else {
+            hipLaunchKernelGGL(k_red_band<double>, dim3(maxP, nd), dim3(256), 0, s, g.rd_desc.as<RedDirect>(),
+                               g.rd_bsd.as<BandSys>(), (const double *)g.vt.p, (const double *)g.vb.p,
+                               (const double *)g.wt.p, (const double *)g.wb.p, g.b, g.rd_ab.as<double>());
+            hipLaunchKernelGGL(k_gbtrf<double>, dim3(nd), dim3(GB_THREADS), lds, s, g.rd_bsd.as<BandSys>(),
+                               g.rd_ab.as<double>(), g.rd_piv.as<int>(), g.rd_info.as<int>());
+        }
+        HIPCK(hipGetLastError());
+        std::vector<int> info(nd);
+        if ((e = get_small({{info.data(), g.rd_info.p, nd * sizeof(int)}}))) return e;
+        for (int q = 0; q < nd; ++q)
+            if (info[q]) {
+                set_err("singular reduced system (zero pivot in column %ld of R)", (long)(info[q] - 1));
+                return SPIKE_ERR_SINGULAR;
+            }
+        g.rd_factored = true;
+    }
+    // right-hand sides of this pass for the systems in `which`, band solve, scatter into the iterate
+    int64_t xe = 0;
+    int maxnr = 1, nrow = 0;
+    std::vector<int> items;
+    std::vector<int> items_g;
+    std::vector<RedDirect> pass;
+    const size_t lds_cap = 48 * 1024;
+    size_t lds = 0;
+    for (int q = 0; q < nd; ++q) {
+        if (std::find(which.begin(), which.end(), g.rd_sys[q]) == which.end()) continue;
+        pass.push_back(g.rd_desc_h[q]);
+        BandSys &B = g.rd_bs[q];
+        B.m = ymax; B.ldx = B.n; B.x = xe;
+        xe += (int64_t)B.n * ymax;
+        maxnr = std::max(maxnr, B.n);
+        nrow = std::max(nrow, (g.rd_desc_h[q].P - 1) * (g.rd_desc_h[q].ku + g.rd_desc_h[q].kl) * ymax);
+        const size_t per_col = (size_t)B.n * hs;
+        if (per_col <= lds_cap) {
+            const int mc = (int)std::min<size_t>({64, (size_t)ymax, lds_cap / per_col});
+            for (int c = 0; c < ymax; c += mc) items.insert(items.end(), {q, c, std::min(mc, ymax - c)});
+            lds = std::max(lds, per_col * mc);
+        } else {
+            for (int c = 0; c < ymax; c += 64) items_g.insert(items_g.end(), {q, c, std::min(64, ymax - c)});
+        }
+    }
+    const int npass = (int)pass.size();
+    if (!npass) return SPIKE_SUCCESS;
+    if ((e = g.rd_x.ensure((size_t)xe * hs))) return e;
+    if ((e = upload(g.rd_bsd, g.rd_bs.data(), nd * sizeof(BandSys), s)) ||
+        (e = upload(g.rd_pass, pass.data(), npass * sizeof(RedDirect), s)))
+        return e;
+    std::vector<int> all(items);
+    all.insert(all.end(), items_g.begin(), items_g.end());
+    if ((e = upload(g.rd_items, all.data(), all.size() * sizeof(int), s))) return e;
+    const unsigned rgrid = (unsigned)std::min<int64_t>(((int64_t)maxnr * ymax + 255) / 256, 4096);
+    const unsigned sgrid = (unsigned)std::min<int64_t>(((int64_t)nrow + 255) / 256 + 1, 4096);
+    const int ni = (int)items.size() / 3, ng = (int)items_g.size() / 3;
+    if (dtype == SPIKE_DTYPE_Z) {
+        hipLaunchKernelGGL(k_red_rhs<zd>, dim3(rgrid, npass), dim3(256), 0, s, g.rd_pass.as<RedDirect>(), g.rd_bsd.as<BandSys>(),
+                           (const zd *)g.yt.p, (const zd *)g.yb.p, g.b, ymax, g.rd_x.as<zd>());
+        if (ni) hipLaunchKernelGGL((k_gbtrs<zd, true>), dim3(ni), dim3(256), lds, s, g.rd_bsd.as<BandSys>(), g.rd_items.as<int>(),
+                                   (const zd *)g.rd_ab.p, (const int *)g.rd_piv.p, g.rd_x.as<zd>());
+        if (ng) hipLaunchKernelGGL((k_gbtrs<zd, false>), dim3(ng), dim3(256), 0, s, g.rd_bsd.as<BandSys>(),
+                                   g.rd_items.as<int>() + 3 * ni, (const zd *)g.rd_ab.p, (const int *)g.rd_piv.p,
+                                   g.rd_x.as<zd>());
+        hipLaunchKernelGGL(k_red_scatter<zd>, dim3(sgrid, npass), dim3(256), 0, s, g.rd_pass.as<RedDirect>(),
+                           g.rd_bsd.as<BandSys>(), (const zd *)g.rd_x.p, g.b, ymax, xo.as<zd>());
+    } else {
+        hipLaunchKernelGGL(k_red_rhs<double>, dim3(rgrid, npass), dim3(256), 0, s, g.rd_pass.as<RedDirect>(),
+                           g.rd_bsd.as<BandSys>(), (const double *)g.yt.p, (const double *)g.yb.p, g.b, ymax,
+                           g.rd_x.as<double>());
+        if (ni) hipLaunchKernelGGL((k_gbtrs<double, true>), dim3(ni), dim3(256), lds, s, g.rd_bsd.as<BandSys>(),
+                                   g.rd_items.as<int>(), (const double *)g.rd_ab.p, (const int *)g.rd_piv.p,
+                                   g.rd_x.as<double>());
+        if (ng) hipLaunchKernelGGL((k_gbtrs<double, false>), dim3(ng), dim3(256), 0, s, g.rd_bsd.as<BandSys>(),
+                                   g.rd_items.as<int>() + 3 * ni, (const double *)g.rd_ab.p, (const int *)g.rd_piv.p,
+                                   g.rd_x.as<double>());
+        hipLaunchKernelGGL(k_red_scatter<double>, dim3(sgrid, npass), dim3(256), 0, s, g.rd_pass.as<RedDirect>(),
+                           g.rd_bsd.as<BandSys>(), (const double *)g.rd_x.p, g.b, ymax, xo.as<double>());
+    }
+    HIPCK(hipGetLastError());
+    return SPIKE_SUCCESS;
+}
+
 // ============================================================================ solve
 int Engine::solve_chunk(int c0, int ymax) {
     hipStream_t s = stream;
@@ -902,14 +1053,30 @@ int Engine::solve_chunk(int c0, int ymax) {
             const size_t slot_elems = (size_t)2 * g.b * ymax;
             if (dtype == SPIKE_DTYPE_Z) { if ((e = fill_zero<zd>(g.red0, nif * slot_elems, s))) return e; }
             else if ((e = fill_zero<double>(g.red0, nif * slot_elems, s))) return e;
-            std::vector<int> ract = active[gi];
+            // systems of this pass on the direct band LU: forced by the options, or switched
+            // there because their block-Jacobi sweeps did not contract earlier in this factorization
+            std::vector<int> ract = active[gi], direct;
+            for (size_t gs = 0; gs < ngs; ++gs) {
+                if (!ract[gs]) continue;
+                SysHost &S = sys[g.sys[gs]];
+                if (S.ku + S.kl == 0) continue;
+                if (o.reduced_solver == SPIKE_RED_DIRECT || S.red_direct) {
+                    direct.push_back((int)gs);
+                    ract[gs] = 0;
+                    S.red_direct = true;
+                    S.red_iters = 0;
+                    S.red_update = 0.0;
+                }
+            }
+            const bool auto_direct = o.reduced_solver == SPIKE_RED_AUTO;
             if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
             DevBuf *xo = &g.red0, *xn = &g.red1;
             std::vector<double> conv(nif * ncc * 2);
             std::vector<double> dmax(ngs), vmax(ngs);
             int iters = 0;
             const int maxit = std::max(2, o.reduced_maxit);
-            while (iters < maxit) {
+            const bool any_jacobi = std::any_of(ract.begin(), ract.end(), [](int a) { return a != 0; });
+            while (any_jacobi && iters < maxit) {
                 // two sweeps per convergence check (the first sweep from zero is the truncated
                 // solution; the second measures the coupling); systems stop once converged
                 for (int rep = 0; rep < 2 && iters < maxit; ++rep) {
@@ -951,20 +1118,26 @@ int Engine::solve_chunk(int c0, int ymax) {
                     dmax[gs] = std::max(dmax[gs], conv[2 * q]);
                     vmax[gs] = std::max(vmax[gs], conv[2 * q + 1]);
                 }
-                bool more = false;
                 for (size_t gs = 0; gs < ngs; ++gs) {
                     if (!ract[gs]) continue;
                     SysHost &S = sys[g.sys[gs]];
                     const double rel = vmax[gs] > 0 ? dmax[gs] / vmax[gs] : dmax[gs];
                     S.red_iters = iters;
                     S.red_update = rel;
-                    if (!(rel <= rtol)) more = true;
-                    else ract[gs] = 0;
-                    if (!std::isfinite(rel) || rel > 1e3) { ract[gs] = 0; S.red_update = rel; }
+                    if (rel <= rtol) ract[gs] = 0;                  // converged
+                    else if (!std::isfinite(rel) || rel > 1e3) {    // diverging
+                        ract[gs] = 0;
+                        if (auto_direct) { direct.push_back((int)gs); S.red_direct = true; }
+                    }
                 }
-                if (!more) break;
+                if (!std::any_of(ract.begin(), ract.end(), [](int a) { return a != 0; })) break;
+                if (iters >= maxit) break;
                 if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
             }
+            if (auto_direct)                                         // not converged within maxit sweeps
+                for (size_t gs = 0; gs < ngs; ++gs)
+                    if (ract[gs]) { direct.push_back((int)gs); sys[g.sys[gs]].red_direct = true; }
+            if (!direct.empty() && (e = reduced_direct(g, direct, *xo, ymax))) return e;
             redcur[gi] = xo;     // last written iterate
             g.red_last = (xo == &g.red1) ? 1 : 0;
         }
@@ -1071,6 +1244,8 @@ void fill_stats(const Engine &E, spike_stats_t *s) {
     s->precision = E.o.precision;
     s->dtype = E.dtype;
     s->device_bytes = E.device_bytes();
+    s->reduced_direct = 0;
+    for (auto &X : E.sys) s->reduced_direct += X.red_direct ? 1 : 0;
 }
 
 }  // namespace
@@ -1093,6 +1268,7 @@ void spike_default_opts(spike_opts_t *o) {
     o->reduced_tol = 1e-15;
     o->sweep_waves = 0;
     o->verbose = 0;
+    o->reduced_solver = SPIKE_RED_AUTO;
 }
 
 const char *spike_error_string(int code) {
@@ -1421,13 +1597,148 @@ int spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s, double *rel
 }
 
 // ============================================================================ residual helper
+}  // extern "C"
+
+namespace {
+
+// Per-column ||b - A x|| / ||b|| of device arrays (column-major, leading dimension n) with the
+// residual r = b - A x left in d_r; small staging buffers come from `ws` (kept by the caller).
+struct ResidualWs {
+    DevBuf dch, dsys, dpart, dnorm, dc0;
+};
+
+int residual_dev(int dtype, int n, const int *d_rowptr, const int *d_colind, const void *d_aij, int nrhs, const void *d_x,
+                 const void *d_b, void *d_r, double *rel, ResidualWs &ws, hipStream_t st) {
+    const size_t hs = h_bytes(dtype);
+    std::vector<RowChunk> ch;
+    for (int r = 0; r < n; r += RES_ROWS) ch.push_back({0, r});
+    const int nch = (int)ch.size();
+    int e;
+    if ((e = upload(ws.dch, ch.data(), ch.size() * sizeof(RowChunk), st)) || (e = ws.dpart.ensure((size_t)nch * 64 * 2 * 8)) ||
+        (e = ws.dnorm.ensure(64 * 2 * 8)))
+        return e;
+    for (int c0 = 0; c0 < nrhs; c0 += 64) {
+        const int m = std::min(64, nrhs - c0);
+        SysDev sd;
+        memset(&sd, 0, sizeof sd);
+        sd.rowptr = d_rowptr; sd.colind = d_colind; sd.aij = d_aij; sd.n = n; sd.m = m;
+        sd.ldb = n; sd.ldx = n; sd.ldr = n;
+        sd.bptr = (const char *)d_b + (size_t)c0 * n * hs;
+        sd.xptr = (char *)d_x + (size_t)c0 * n * hs;
+        sd.rptr = (char *)d_r + (size_t)c0 * n * hs;
+        int c0s[2] = {0, nch};
+        if ((e = upload(ws.dsys, &sd, sizeof sd, st)) || (e = upload(ws.dc0, c0s, sizeof c0s, st))) return e;
+        if ((e = launch_residual(dtype, m, 64, false, (unsigned)nch, st, ws.dch.as<RowChunk>(), ws.dsys.as<SysDev>(),
+                                 ws.dpart.as<double>(), (const int *)nullptr)))
+            return e;
+        hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, st, ws.dc0.as<int>(), ws.dsys.as<SysDev>(),
+                           ws.dpart.as<double>(), ws.dnorm.as<double>(), 64);
+        HIPCK(hipGetLastError());
+        double nr[128];
+        HIPCK(hipMemcpyAsync(nr, ws.dnorm.p, sizeof nr, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        for (int c = 0; c < m; ++c) rel[c0 + c] = nr[2 * c + 1] > 0 ? sqrt(nr[2 * c] / nr[2 * c + 1]) : sqrt(nr[2 * c]);
+    }
+    return SPIKE_SUCCESS;
+}
+
+template <typename H>
+int axpy_dev(void *x, const void *d, size_t count, hipStream_t st) {
+    if (!count) return 0;
+    const unsigned grid = (unsigned)std::min<size_t>((count + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_add<H>, dim3(grid), dim3(256), 0, st, (H *)x, (const H *)d, (int64_t)count);
+    HIPCK(hipGetLastError());
+    return 0;
+}
+
+// Band solve of one matrix on the device: LU with partial pivoting of its band, then gbtrs on
+// the panel x (n x nrhs, leading dimension n) in place.
+struct BandOne {
+    DevBuf ab, piv, info, sysd, items;
+    BandSys B{};
+    size_t lds_f = 0;
+};
+
+int band_factor_dev(int dtype, int n, int ku, int kl, const int *d_rowptr, const int *d_colind, const void *d_aij,
+                    BandOne &F, hipStream_t st) {
+    const size_t hs = h_bytes(dtype);
+    F.B = BandSys{};
+    F.B.n = n; F.B.kl = kl; F.B.ku = ku; F.B.ldab = 2 * kl + ku + 1;
+    F.lds_f = (size_t)(std::max(kl, 1) + kl + ku + 1) * hs;
+    if (F.lds_f > 64 * 1024) {
+        set_err("band too wide for the device band LU (kl %ld, ku %ld)", (long)kl, (long)ku);
+        return SPIKE_ERR_UNSUPPORTED;
+    }
+    const size_t elems = (size_t)n * F.B.ldab;
+    int e;
+    if ((e = F.ab.ensure(elems * hs)) || (e = F.piv.ensure((size_t)n * sizeof(int))) || (e = F.info.ensure(sizeof(int))) ||
+        (e = upload(F.sysd, &F.B, sizeof F.B, st)))
+        return e;
+    const unsigned rg = (unsigned)((n + 255) / 256);
+    if (dtype == SPIKE_DTYPE_Z) {
+        if ((e = fill_zero<zd>(F.ab, elems, st))) return e;
+        hipLaunchKernelGGL(k_csr_to_band<zd>, dim3(rg), dim3(256), 0, st, d_rowptr, d_colind, (const zd *)d_aij, n, kl, ku,
+                           F.B.ldab, F.ab.as<zd>());
+        hipLaunchKernelGGL(k_gbtrf<zd>, dim3(1), dim3(GB_THREADS), F.lds_f, st, F.sysd.as<BandSys>(), F.ab.as<zd>(),
+                           F.piv.as<int>(), F.info.as<int>());
+    } else {
+        if ((e = fill_zero<double>(F.ab, elems, st))) return e;
+        hipLaunchKernelGGL(k_csr_to_band<double>, dim3(rg), dim3(256), 0, st, d_rowptr, d_colind, (const double *)d_aij, n,
+                           kl, ku, F.B.ldab, F.ab.as<double>());
+        hipLaunchKernelGGL(k_gbtrf<double>, dim3(1), dim3(GB_THREADS), F.lds_f, st, F.sysd.as<BandSys>(), F.ab.as<double>(),
+                           F.piv.as<int>(), F.info.as<int>());
+    }
+    HIPCK(hipGetLastError());
+    int info = 0;
+    HIPCK(hipMemcpyAsync(&info, F.info.p, sizeof info, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (info) {
+        set_err("singular matrix (no nonzero pivot in column %ld)", (long)(info - 1));
+        return SPIKE_ERR_SINGULAR;
+    }
+    return SPIKE_SUCCESS;
+}
+
+int band_solve_dev(int dtype, BandOne &F, int nrhs, void *d_x, hipStream_t st) {
+    const size_t hs = h_bytes(dtype);
+    const int n = F.B.n;
+    F.B.m = nrhs; F.B.ldx = n; F.B.x = 0;
+    const size_t per_col = (size_t)n * hs, cap = 48 * 1024;
+    const bool lx = per_col <= cap;
+    const int mc = lx ? (int)std::min<size_t>({64, (size_t)nrhs, cap / per_col}) : std::min(64, nrhs);
+    std::vector<int> items;
+    for (int c = 0; c < nrhs; c += mc) items.insert(items.end(), {0, c, std::min(mc, nrhs - c)});
+    int e;
+    if ((e = upload(F.sysd, &F.B, sizeof F.B, st)) || (e = upload(F.items, items.data(), items.size() * sizeof(int), st))) return e;
+    const unsigned ni = (unsigned)(items.size() / 3);
+    const size_t lds = lx ? per_col * mc : 0;
+    if (dtype == SPIKE_DTYPE_Z) {
+        if (lx) hipLaunchKernelGGL((k_gbtrs<zd, true>), dim3(ni), dim3(256), lds, st, F.sysd.as<BandSys>(), F.items.as<int>(),
+                                   (const zd *)F.ab.p, (const int *)F.piv.p, (zd *)d_x);
+        else hipLaunchKernelGGL((k_gbtrs<zd, false>), dim3(ni), dim3(256), 0, st, F.sysd.as<BandSys>(), F.items.as<int>(),
+                                (const zd *)F.ab.p, (const int *)F.piv.p, (zd *)d_x);
+    } else {
+        if (lx) hipLaunchKernelGGL((k_gbtrs<double, true>), dim3(ni), dim3(256), lds, st, F.sysd.as<BandSys>(),
+                                   F.items.as<int>(), (const double *)F.ab.p, (const int *)F.piv.p, (double *)d_x);
+        else hipLaunchKernelGGL((k_gbtrs<double, false>), dim3(ni), dim3(256), 0, st, F.sysd.as<BandSys>(), F.items.as<int>(),
+                                (const double *)F.ab.p, (const int *)F.piv.p, (double *)d_x);
+    }
+    HIPCK(hipGetLastError());
+    return SPIKE_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
 int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind, const void *aij, int nrhs,
                         const void *x, const void *b, double *rel) {
     if (n <= 0 || nnz < 0 || nrhs < 0 || !rowptr || !rel || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
         return SPIKE_ERR_INVALID_ARG;
     if (nrhs == 0) return SPIKE_SUCCESS;
     const size_t hs = h_bytes(dtype);
-    DevBuf drp, dci, da, dx, db, dr, dch, dsys, dpart, dnorm, dc0;
+    DevBuf drp, dci, da, dx, db, dr;
+    ResidualWs ws;
     int e;
     if ((e = drp.ensure((size_t)(n + 1) * 4)) || (e = dci.ensure((size_t)nnz * 4)) || (e = da.ensure((size_t)nnz * hs)) ||
         (e = dx.ensure((size_t)n * nrhs * hs)) || (e = db.ensure((size_t)n * nrhs * hs)) || (e = dr.ensure((size_t)n * nrhs * hs)))
@@ -1439,32 +1750,58 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
     }
     HIPCK(hipMemcpy(dx.p, x, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(db.p, b, (size_t)n * nrhs * hs, hipMemcpyHostToDevice));
-    std::vector<RowChunk> ch;
-    for (int r = 0; r < n; r += RES_ROWS) ch.push_back({0, r});
-    const int nch = (int)ch.size();
-    for (int c0 = 0; c0 < nrhs; c0 += 64) {
-        const int m = std::min(64, nrhs - c0);
-        SysDev sd;
-        memset(&sd, 0, sizeof sd);
-        sd.rowptr = drp.as<int>(); sd.colind = dci.as<int>(); sd.aij = da.p; sd.n = n; sd.m = m;
-        sd.ldb = n; sd.ldx = n; sd.ldr = n;
-        sd.bptr = (char *)db.p + (size_t)c0 * n * hs;
-        sd.xptr = (char *)dx.p + (size_t)c0 * n * hs;
-        sd.rptr = (char *)dr.p + (size_t)c0 * n * hs;
-        int c0s[2] = {0, nch};
-        if ((e = upload(dch, ch.data(), ch.size() * sizeof(RowChunk), 0)) || (e = upload(dsys, &sd, sizeof sd, 0)) ||
-            (e = upload(dc0, c0s, sizeof c0s, 0)) || (e = dpart.ensure((size_t)nch * 64 * 2 * 8)) || (e = dnorm.ensure(64 * 2 * 8)))
-            return e;
-        if ((e = launch_residual(dtype, m, 64, false, (unsigned)nch, 0, dch.as<RowChunk>(), dsys.as<SysDev>(), dpart.as<double>(),
-                                 (const int *)nullptr)))
-            return e;
-        hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, 0, dc0.as<int>(), dsys.as<SysDev>(), dpart.as<double>(),
-                           dnorm.as<double>(), 64);
-        HIPCK(hipGetLastError());
-        double nr[128];
-        HIPCK(hipMemcpy(nr, dnorm.p, sizeof nr, hipMemcpyDeviceToHost));
-        for (int c = 0; c < m; ++c) rel[c0 + c] = nr[2 * c + 1] > 0 ? sqrt(nr[2 * c] / nr[2 * c + 1]) : sqrt(nr[2 * c]);
+    return residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rel, ws, 0);
+}
+
+// Direct solve of a banded CSR matrix by the device band LU with partial pivoting, refined in
+// FP64 (x += A^{-1} (b - A x)) while the largest relative residual keeps shrinking (up to three
+// corrections, the reference's PARDISO setting for R, spike_algebra.c:299).
+int spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind, const void *aij, int nrhs,
+                          const void *b, int ldb, void *x, int ldx, double *rel) {
+    if (n <= 0 || nnz < 0 || nnz > INT32_MAX || nrhs < 0 || !rowptr || (nnz > 0 && (!colind || !aij)) ||
+        (nrhs > 0 && (!b || !x)) || ldb < n || ldx < n || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
+        return SPIKE_ERR_INVALID_ARG;
+    if (matrix_CheckCSR(n, (uLong_t)nnz, colind, rowptr) != SPIKE_SUCCESS) return SPIKE_ERR_MALFORMED_CSR;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); set_err("no HIP device%s", ""); return SPIKE_ERR_DEVICE; }
+    integer_t ku = 0, kl = 0;
+    matrix_ComputeBandwidth(n, colind, rowptr, nullptr, &ku, &kl);
+    const size_t hs = h_bytes(dtype);
+    DevBuf drp, dci, da, dx, db, dr;
+    int e;
+    if ((e = drp.ensure((size_t)(n + 1) * 4)) || (e = dci.ensure((size_t)std::max<int64_t>(nnz, 1) * 4)) ||
+        (e = da.ensure((size_t)std::max<int64_t>(nnz, 1) * hs)))
+        return e;
+    HIPCK(hipMemcpy(drp.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCK(hipMemcpy(dci.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(da.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
     }
+    BandOne F;
+    if ((e = band_factor_dev(dtype, n, ku, kl, drp.as<int>(), dci.as<int>(), da.p, F, 0))) return e;
+    if (nrhs == 0) return SPIKE_SUCCESS;
+    const size_t panel = (size_t)n * nrhs * hs;
+    if ((e = dx.ensure(panel)) || (e = db.ensure(panel)) || (e = dr.ensure(panel))) return e;
+    HIPCK(hipMemcpy2D(db.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(dx.p, db.p, panel, hipMemcpyDeviceToDevice));
+    if ((e = band_solve_dev(dtype, F, nrhs, dx.p, 0))) return e;
+    std::vector<double> rl(nrhs);
+    ResidualWs ws;
+    if ((e = residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rl.data(), ws, 0))) return e;
+    double worst = *std::max_element(rl.begin(), rl.end());
+    for (int it = 0; it < 3 && worst > 0.0 && std::isfinite(worst); ++it) {
+        if ((e = band_solve_dev(dtype, F, nrhs, dr.p, 0))) return e;          // delta = A^{-1} r, in place
+        if (dtype == SPIKE_DTYPE_Z) { if ((e = axpy_dev<zd>(dx.p, dr.p, (size_t)n * nrhs, 0))) return e; }
+        else if ((e = axpy_dev<double>(dx.p, dr.p, (size_t)n * nrhs, 0))) return e;
+        std::vector<double> rn(nrhs);
+        if ((e = residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rn.data(), ws, 0))) return e;
+        const double wn = *std::max_element(rn.begin(), rn.end());
+        rl = rn;
+        if (!(wn < 0.5 * worst)) break;               // stagnating: the correction is at rounding level
+        worst = wn;
+    }
+    HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, dx.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
+    if (rel) for (int c = 0; c < nrhs; ++c) rel[c] = rl[c];
     return SPIKE_SUCCESS;
 }
 

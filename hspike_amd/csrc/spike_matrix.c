@@ -58,7 +58,7 @@ matrix_t *matrix_LoadCSR(const char *filename)
     if (fread(M->rowptr, sizeof(integer_t), (size_t)M->n + 1, f) != (size_t)M->n + 1) goto fail;
     fclose(f);
     if (matrix_CheckCSR(M->n, M->nnz, M->colind, M->rowptr) != SPIKE_SUCCESS) { matrix_Deallocate(M); return NULL; }
-    matrix_ComputeBandwidth(M->n, M->colind, M->rowptr, &M->ku, &M->kl);
+    matrix_ComputeBandwidth(M->n, M->colind, M->rowptr, M->aij, &M->ku, &M->kl);
     M->type = _DIAG_BLOCK_;
     return M;
 fail:
@@ -113,7 +113,7 @@ matrix_z_t *matrix_LoadCSR_zl(const char *filename, int planar)
         M->planar_file = planar ? 1 : 0;
     }
     free(raw);
-    matrix_ComputeBandwidth(M->n, M->colind, M->rowptr, &M->ku, &M->kl);
+    matrix_ComputeBandwidth(M->n, M->colind, M->rowptr, M->aij, &M->ku, &M->kl);
     return M;
 fail:
     if (f) fclose(f);
@@ -154,7 +154,7 @@ matrix_t *matrix_CreateFromComponents(integer_t n, uLong_t nnz, integer_t *colin
 {
     matrix_t *A = (matrix_t *)xcalloc(1, sizeof(matrix_t));
     A->n = n; A->nnz = nnz; A->colind = colind; A->rowptr = rowptr; A->aij = aij;
-    matrix_ComputeBandwidth(n, colind, rowptr, &A->ku, &A->kl);
+    matrix_ComputeBandwidth(n, colind, rowptr, aij, &A->ku, &A->kl);
     return A;
 }
 
@@ -169,8 +169,11 @@ Bool_t matrix_AreEqual(const matrix_t *A, const matrix_t *B)
 }
 
 /* ------------------------------------------------------------------ bandwidth / extraction */
-Error_t matrix_ComputeBandwidth(integer_t n, const integer_t *colind, const integer_t *rowptr, integer_t *ku, integer_t *kl)
+Error_t matrix_ComputeBandwidth(integer_t n, const integer_t *colind, const integer_t *rowptr, const double *aij,
+                                integer_t *ku, integer_t *kl)
 {
+    (void)aij;
+    if (n < 0 || !rowptr || !ku || !kl || (n > 0 && rowptr[n] > 0 && !colind)) return SPIKE_ERR_INVALID_ARG;
     integer_t u = 0, l = 0;
     for (integer_t r = 0; r < n; ++r)
         for (integer_t idx = rowptr[r]; idx < rowptr[r + 1]; ++idx) {
@@ -200,7 +203,7 @@ matrix_t *matrix_ExtractMatrix(const matrix_t *M, integer_t r0, integer_t rf, in
         B->rowptr[r - r0 + 1] = (integer_t)q;
     }
     B->type = type;
-    matrix_ComputeBandwidth(B->n, B->colind, B->rowptr, &B->ku, &B->kl);
+    matrix_ComputeBandwidth(B->n, B->colind, B->rowptr, B->aij, &B->ku, &B->kl);
     return B;
 }
 
@@ -259,6 +262,14 @@ sm_schedule_t *spike_solve_analysis_p(const matrix_t *A, integer_t nrhs, integer
     return S;
 }
 
+sm_schedule_t *spike_solve_analysis(matrix_t *A, const integer_t nrhs)
+{
+    if (!A || A->n <= 0) return NULL;
+    if (matrix_ComputeBandwidth(A->n, A->colind, A->rowptr, A->aij, &A->ku, &A->kl) != SPIKE_SUCCESS) return NULL;
+    const char *v = getenv("HSPIKE_PARTITIONS");
+    return spike_solve_analysis_p(A, nrhs, (v && atoi(v) > 0) ? atoi(v) : 4);
+}
+
 void schedule_Destroy(sm_schedule_t *S)
 {
     if (!S) return;
@@ -271,8 +282,9 @@ void schedule_Destroy(sm_schedule_t *S)
  * columns [nr_p - kl, nr_p) when p > 0, a unit diagonal, and ku V entries at columns
  * [nr_{p+1}, nr_{p+1} + ku) when p < P-1.  The first ku rows take the top tips, the last kl
  * the bottom tips. */
-matrix_t *matrix_CreateEmptyReducedSystem(integer_t P, const integer_t *ku, const integer_t *kl)
+matrix_t *matrix_CreateEmptyReducedSystem(integer_t P, const integer_t *n, const integer_t *ku, const integer_t *kl)
 {
+    (void)n;
     if (P <= 0 || !ku || !kl) return NULL;
     integer_t rows = 0;
     uLong_t nnz = 0;
@@ -298,7 +310,7 @@ matrix_t *matrix_CreateEmptyReducedSystem(integer_t P, const integer_t *ku, cons
         nrp = nrp1;
     }
     R->type = _DIAG_BLOCK_;
-    matrix_ComputeBandwidth(R->n, R->colind, R->rowptr, &R->ku, &R->kl);
+    matrix_ComputeBandwidth(R->n, R->colind, R->rowptr, R->aij, &R->ku, &R->kl);
     return R;
 }
 

@@ -45,7 +45,18 @@ typedef struct {
     double reduced_tol;    /* relative update that ends the reduced iteration (1e-15)      */
     int    sweep_waves;    /* waves (16 columns each) per sweep workgroup: 0 auto, 1, 2, 4 */
     int    verbose;        /* 1 = stage timings on stderr                                   */
+    int    reduced_solver; /* SPIKE_RED_AUTO (default), SPIKE_RED_JACOBI or SPIKE_RED_DIRECT  */
 } spike_opts_t;
+
+/* Solver of the reduced system R (stage 3).  AUTO runs block-Jacobi sweeps and switches a
+ * system to the direct solver when its sweeps stop contracting (update growing past 1e3 times
+ * the iterate, non-finite, or not converged within reduced_maxit sweeps); DIRECT assembles R in
+ * the reference's pattern and factors it with a banded LU with partial pivoting on the GPU (the
+ * reference's PARDISO solve of R, spike_algebra.c:263-420); the factors of R stay resident for
+ * the refinement passes and later solves until the matrix is factored again. */
+#define SPIKE_RED_AUTO   0
+#define SPIKE_RED_JACOBI 1
+#define SPIKE_RED_DIRECT 2
 
 typedef struct {
     int     n, ku, kl, b, P, nrhs;
@@ -69,6 +80,7 @@ typedef struct {
     double  ms_factor_total;   /* spike_factor wall time                                     */
     double  ms_solve_total;    /* spike_solve_dev wall time                                  */
     size_t  device_bytes;      /* HBM held by the handle                                     */
+    int     reduced_direct;    /* systems whose reduced system the direct band LU solved      */
 } spike_stats_t;
 
 typedef struct spike_handle spike_handle_t;
@@ -149,6 +161,17 @@ int  spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int p
 /* Relative residual ||A x - b|| / ||b|| of every column (host arrays, FP64 on the GPU). */
 int  spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,
                          const void *aij, int nrhs, const void *x, const void *b, double *rel);
+
+/* Direct solve of a banded CSR matrix (host arrays) on the GPU: the band (ku, kl from the
+ * stored entries) is factored by LU with partial pivoting (LAPACK gbtrf semantics: largest
+ * modulus in the column, first on ties), the columns are solved, and FP64 refinement steps
+ * follow while the largest relative residual halves (at most three).  rel (may be NULL)
+ * receives every column's final ||b - A x|| / ||b||.  kl + (kl + ku) must stay below 4096
+ * (FP64) or 2048 (complex).  Returns SPIKE_ERR_SINGULAR when a column has no nonzero pivot.
+ * Backs directSolver_Host_Solve (the reduced-system solver slot, spike_algebra.h:157-164). */
+int  spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,
+                           const void *aij, int nrhs, const void *b, int ldb, void *x, int ldx,
+                           double *rel);
 
 /* ---------------- request server: continuous batching, memory admission, resident factors --
  * Requests (host arrays, kept alive and unmodified by the caller until spike_server_wait returns)

@@ -11,14 +11,24 @@
  *   - CSR int32 0-based, x and b dense column-major with leading dimension n, host memory;
  *   - the factorization of A is kept for any number of later SolveForRHS calls.
  * Changes: values are FP64 (the reference's complex_t under _DATATYPE_D_); the factors live
- * in HBM; failures return negative SPIKE_ERR_* codes instead of aborting; the handle owns
- * copies of the CSR arrays (the reference keeps the caller's pointers, spike_pardiso.c:93-97,
- * which this backend never needs after Factorize returns).
+ * in HBM; failures return negative SPIKE_ERR_* codes (spike_matrix.h) instead of aborting; the
+ * handle owns copies of the CSR arrays (the reference keeps the caller's pointers,
+ * spike_pardiso.c:93-97, which this backend never needs after Factorize returns).
+ *
+ * The header is self-contained so that it can take a backend's place in the reference's own
+ * translation units (spike_algebra.h:30-36 includes it after spike_datatypes.h): it declares
+ * no container types, its scalar typedefs are the reference's (int) and are skipped when
+ * spike_datatypes.h or this library's spike_matrix.h already made them, and the handle is an
+ * opaque struct.
  */
 #ifndef HSPIKE_SPIKE_HIP_H
 #define HSPIKE_SPIKE_HIP_H
 
-#include "spike_matrix.h"
+#if !defined(_SPIKE_DATATYPES_H_) && !defined(HSPIKE_SPIKE_MATRIX_H) && !defined(HSPIKE_SCALAR_TYPES)
+#define HSPIKE_SCALAR_TYPES
+typedef int integer_t;   /* spike_datatypes.h:86 */
+typedef int Error_t;     /* spike_datatypes.h:87 */
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -51,7 +61,8 @@ Error_t directSolver_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t
                            integer_t *rowptr, double *aij, double *xij, double *bij);
 
 /* spike_algebra.h:157-164: solver for the reduced system R (CSR, FP64) used by the driver's
- * stage 3; here R is solved on the GPU as a banded system. */
+ * stage 3 (spike_interfaces.c:245); here any banded CSR matrix is factored on the GPU by a band
+ * LU with partial pivoting and refined in FP64 (spike_band_solve_host). */
 Error_t directSolver_Host_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t *colind,
                                 integer_t *rowptr, double *aij, double *xij, double *bij);
 

@@ -24,8 +24,11 @@
 extern "C" {
 #endif
 
+#ifndef HSPIKE_SCALAR_TYPES
+#define HSPIKE_SCALAR_TYPES
 typedef int      integer_t;   /* spike_datatypes.h:86 */
 typedef int      Error_t;     /* spike_datatypes.h:87 */
+#endif
 typedef int      Bool_t;      /* spike_datatypes.h:88 */
 typedef uint64_t uLong_t;     /* spike_datatypes.h:91 */
 
@@ -115,8 +118,10 @@ Bool_t      matrix_AreEqual(const matrix_t *A, const matrix_t *B);         /* sp
 Error_t     matrix_CheckCSR(integer_t n, uLong_t nnz, const integer_t *colind, const integer_t *rowptr);
 
 /* ---- bandwidth and extraction */
+/* spike_algebra.h:114-119 / spike_algebra.c:56-95: aij is accepted (reference signature) and not
+ * read; ku = max(col - row), kl = max(row - col) over the stored entries */
 Error_t     matrix_ComputeBandwidth(integer_t n, const integer_t *colind, const integer_t *rowptr,
-                                    integer_t *ku, integer_t *kl);         /* spike_algebra.c:56-95 */
+                                    const double *aij, integer_t *ku, integer_t *kl);
 matrix_t   *matrix_ExtractMatrix(const matrix_t *M, integer_t r0, integer_t rf,
                                  integer_t c0, integer_t cf, matrixtype_t type); /* spike_matrix.c:563-645 */
 block_t    *matrix_ExtractBlock(const matrix_t *M, integer_t r0, integer_t rf,
@@ -125,17 +130,23 @@ block_t    *block_CreateEmptyBlock(integer_t n, integer_t m, integer_t ku, integ
                                    blocktype_t type, blocksection_t section);    /* spike_matrix.c:745-775 */
 Error_t     block_Deallocate(block_t *B);
 
-/* ---- schedule: same partition boundaries as spike_analysis.c:38-76 with a caller-chosen p
- *      (p <= 0 selects the reference default of 4, spike_analysis.c:168) */
+/* ---- schedule: same partition boundaries as spike_analysis.c:38-76.
+ * spike_solve_analysis (spike_analysis.h:65, spike_analysis.c:3-83) computes A's bandwidth into
+ * A->ku / A->kl as the reference does and uses p = 4 (spike_analysis.c:168), or the value of
+ * the HSPIKE_PARTITIONS environment variable when it is set; spike_solve_analysis_p takes a
+ * caller-chosen p (p <= 0 selects 4) and reads A->ku / A->kl as they are. */
+sm_schedule_t *spike_solve_analysis(matrix_t *A, const integer_t nrhs);
 sm_schedule_t *spike_solve_analysis_p(const matrix_t *A, integer_t nrhs, integer_t p);
-void           schedule_Destroy(sm_schedule_t *S);
+void           schedule_Destroy(sm_schedule_t *S);      /* spike_analysis.c:85-95 */
 
 /* ---- reduced system in the reference's CSR pattern (spike_matrix.c:657-740, 1013-1120,
  *      1254-1332, 1379-1611).  Tips are column-major:
  *        Vt[ku x ku], Vb[kl x ku], Wt[ku x kl], Wb[kl x kl], yt[ku x nrhs], yb[kl x nrhs]
  *      per partition, each array a concatenation over partitions (V absent for p = P-1, W for p = 0
  *      but slots are still reserved).  Returns R (N_r x N_r CSR) and writes g (N_r x nrhs). */
-matrix_t   *matrix_CreateEmptyReducedSystem(integer_t P, const integer_t *ku, const integer_t *kl);
+/* spike_matrix.h:242-245: n (the partition row bounds, P+1 entries) is accepted and not read,
+ * as in the reference; the pattern depends on ku[p], kl[p] only */
+matrix_t   *matrix_CreateEmptyReducedSystem(integer_t P, const integer_t *n, const integer_t *ku, const integer_t *kl);
 Error_t     matrix_AssembleReducedSystem(matrix_t *R, integer_t P, integer_t ku, integer_t kl,
                                          const double *Vt, const double *Vb,
                                          const double *Wt, const double *Wb);

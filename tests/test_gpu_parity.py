@@ -62,7 +62,7 @@ def test_penta10_reduced_system_matches_golden(hs, oracle_mod):
     import ctypes
     L = hs.load()
     ku_a = np.array([2, 2], np.int32)
-    Rp = L.matrix_CreateEmptyReducedSystem(2, ku_a.ctypes.data_as(ctypes.c_void_p), ku_a.ctypes.data_as(ctypes.c_void_p))
+    Rp = L.matrix_CreateEmptyReducedSystem(2, None, ku_a.ctypes.data_as(ctypes.c_void_p), ku_a.ctypes.data_as(ctypes.c_void_p))
     assert Rp
 
     class M(ctypes.Structure):

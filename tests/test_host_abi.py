@@ -122,7 +122,7 @@ def test_reduced_pattern_matches_oracle(L):
     kua = np.full(P, ku, np.int32)
     kla = np.full(P, kl, np.int32)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    R = L.matrix_CreateEmptyReducedSystem(P, p(kua), p(kla))
+    R = L.matrix_CreateEmptyReducedSystem(P, None, p(kua), p(kla))
     cat = {k: np.ascontiguousarray(info[k].transpose(0, 2, 1)).ravel() for k in ("Vt", "Vb", "Wt", "Wb", "yt", "yb")}
     assert L.matrix_AssembleReducedSystem(R, P, ku, kl, p(cat["Vt"]), p(cat["Vb"]), p(cat["Wt"]), p(cat["Wb"])) == 0
 
@@ -192,3 +192,151 @@ def test_autotuner_is_pure():
     assert 32 <= a <= 8192
     assert hs.autotune_partitions(1000, 2, 2, 1) >= 1
     assert hs.autotune_partitions(100, 60, 60, 1) == 1
+
+
+# ---------------------------------------------------------------- drop-in boundary (headers, memory ABI)
+REF_TU = r"""
+/* the reference's own typedefs and containers, as spike_datatypes.h:29-91 and
+ * spike_matrix.h:35-72 declare them under _DATATYPE_D_, ahead of the backend header
+ * (spike_algebra.h:30-36 includes the backend after them) */
+#define _SPIKE_DATATYPES_H_
+#include <stdint.h>
+typedef double real_t;
+typedef double complex_t;
+typedef int integer_t;
+typedef int Error_t;
+typedef int Bool_t;
+typedef double spike_timer_t;
+typedef uint64_t uLong_t;
+typedef enum { _C_BLOCK_, _DIAG_BLOCK_, _B_BLOCK_ } matrixtype_t;
+typedef struct { integer_t n; uLong_t nnz; integer_t ku; integer_t kl; matrixtype_t type;
+                 integer_t *colind; integer_t *rowptr; complex_t *aij; } matrix_t;
+typedef enum { _V_BLOCK_, _W_BLOCK_, _RHS_BLOCK_ } blocktype_t;
+typedef enum { _TOP_SECTION_, _CENTRAL_SECTION_, _BOTTOM_SECTION_, _WHOLE_SECTION_ } blocksection_t;
+typedef struct { blocktype_t type; blocksection_t section; integer_t n, m, ku, kl; complex_t *aij; } block_t;
+#define _HIP_BACKEND_
+#if defined (_HIP_BACKEND_)
+    #include "spike_hip.h"
+#endif
+/* the reference's prototypes of the functions the plugin also provides (spike_algebra.h:122-164) */
+Error_t ComputeResidualOfLinearSystem(integer_t *restrict colind, integer_t *restrict rowptr,
+                                      complex_t *restrict aij, complex_t *restrict x, complex_t *restrict b,
+                                      integer_t n, integer_t nrhs);
+Error_t directSolver_Host_Solve(integer_t n, integer_t nnz, integer_t nrhs, integer_t *restrict colind,
+                                integer_t *restrict rowptr, complex_t *restrict aij, complex_t *restrict x,
+                                complex_t *restrict b);
+/* a driver's use of the slot (spike_interfaces.c:113-143) */
+int drive(matrix_t *A, block_t *x, block_t *b) {
+    DirectSolverHander_t *h = directSolver_CreateHandler();
+    directSolver_Configure(h, b->m);
+    directSolver_Factorize(h, A->n, (integer_t)A->nnz, A->colind, A->rowptr, A->aij);
+    directSolver_SolveForRHS(h, b->m, x->aij, b->aij);
+    directSolver_ShowStatistics(h);
+    return directSolver_Finalize(h);
+}
+"""
+
+
+@pytest.mark.parametrize("std", ["c99", "gnu11"])
+def test_plugin_header_in_reference_translation_unit(tmp_path, std):
+    """spike_hip.h takes a backend's place after the reference's typedefs (spike_algebra.h:30-36)
+    without redeclaring any of them, and its prototypes agree with the reference's."""
+    import subprocess
+    src = tmp_path / "tu.c"
+    src.write_text(REF_TU)
+    r = subprocess.run(["gcc", "-std=" + std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_public_headers_compile_together(tmp_path):
+    import subprocess
+    src = tmp_path / "all.c"
+    src.write_text('#include "spike_hip.h"\n#include "spike_interfaces.h"\n#include "spike_memory.h"\n'
+                   '#include "spike_ext.h"\n#include "spike_matrix.h"\nint main(void){return 0;}\n')
+    for std in ("c99", "c11"):
+        r = subprocess.run(["gcc", "-std=" + std, "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                            str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++", "-I",
+                        os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_reference_l1_signatures():
+    """matrix_ComputeBandwidth(n, colind, rowptr, aij, ku, kl) (spike_algebra.h:114-119),
+    matrix_CreateEmptyReducedSystem(P, n, ku, kl) (spike_matrix.h:242-245),
+    spike_solve_analysis(A, nrhs) (spike_analysis.h:65), the memory ABI (spike_memory.h:33-37)."""
+    flat = lambda t: re.sub(r"\s+", " ", re.sub(r"/\*.*?\*/", "", t, flags=re.S))
+    m = flat(open(os.path.join(ROOT, "include", "spike_matrix.h")).read())
+    assert re.search(r"Error_t matrix_ComputeBandwidth\(integer_t n, const integer_t \*colind, const integer_t \*rowptr, "
+                     r"const double \*aij, integer_t \*ku, integer_t \*kl\);", m)
+    assert re.search(r"matrix_CreateEmptyReducedSystem\(integer_t P, const integer_t \*n, const integer_t \*ku, "
+                     r"const integer_t \*kl\);", m)
+    assert re.search(r"sm_schedule_t \*spike_solve_analysis\(matrix_t \*A, const integer_t nrhs\);", m)
+    mem = flat(open(os.path.join(ROOT, "include", "spike_memory.h")).read())
+    assert "void *_spike_malloc(const int alignment, const size_t nmemb, const size_t size, const char *function, const int line);" in mem
+    assert "void spike_free(void *ptr);" in mem and "void spike_nullify(void *ptr);" in mem
+    assert "extern unsigned int cnt_alloc;" in mem and "extern unsigned int cnt_free;" in mem
+
+
+def test_bandwidth_and_analysis(L):
+    """Bandwidth over the stored entries (the reference fixture Tests/bandwidth: ku = 2, kl = 4) and
+    the schedule of spike_solve_analysis: p = 4, n[i] = i * floor(n / 4), the remainder to the last
+    partition, r[i] = i (ku + kl) (spike_analysis.c:38-76)."""
+    M = L.matrix_LoadCSR(ref_path("bandwidth/banded.bin").encode())
+    assert M
+
+    class Mat(ctypes.Structure):
+        _fields_ = [("n", ctypes.c_int), ("nnz", ctypes.c_uint64), ("ku", ctypes.c_int), ("kl", ctypes.c_int),
+                    ("type", ctypes.c_int), ("colind", ctypes.c_void_p), ("rowptr", ctypes.c_void_p),
+                    ("aij", ctypes.c_void_p)]
+
+    class Sched(ctypes.Structure):
+        _fields_ = [("p", ctypes.c_int), ("max_n", ctypes.c_int), ("max_m", ctypes.c_int), ("max_nrhs", ctypes.c_int),
+                    ("blockingDistance", ctypes.c_int), ("n", ctypes.POINTER(ctypes.c_int)),
+                    ("r", ctypes.POINTER(ctypes.c_int)), ("ku", ctypes.POINTER(ctypes.c_int)),
+                    ("kl", ctypes.POINTER(ctypes.c_int))]
+    m = ctypes.cast(M, ctypes.POINTER(Mat)).contents
+    ku, kl = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert L.matrix_ComputeBandwidth(m.n, m.colind, m.rowptr, m.aij, ctypes.byref(ku), ctypes.byref(kl)) == 0
+    assert (ku.value, kl.value) == (2, 4)
+    m.ku = m.kl = 0                                     # spike_solve_analysis recomputes them into A
+    old = os.environ.pop("HSPIKE_PARTITIONS", None)
+    try:
+        S = L.spike_solve_analysis(M, 3)
+    finally:
+        if old is not None:
+            os.environ["HSPIKE_PARTITIONS"] = old
+    assert S
+    m = ctypes.cast(M, ctypes.POINTER(Mat)).contents
+    assert (m.ku, m.kl) == (2, 4)
+    s = ctypes.cast(S, ctypes.POINTER(Sched)).contents
+    assert s.p == 4 and s.max_nrhs == 4 and s.blockingDistance == 200
+    assert [s.n[i] for i in range(5)] == [0, 2, 4, 6, 10]
+    assert [s.r[i] for i in range(5)] == [0, 6, 12, 18, 24]
+    L.schedule_Destroy(S)
+    L.matrix_Deallocate(M)
+
+
+def test_memory_abi(L):
+    """_spike_malloc / spike_free / spike_nullify (spike_memory.h:33-37): aligned, counted, and
+    NULL instead of abort() on a request that cannot be met."""
+    cnt_a = ctypes.c_uint.in_dll(L, "cnt_alloc")
+    cnt_f = ctypes.c_uint.in_dll(L, "cnt_free")
+    a0, f0 = cnt_a.value, cnt_f.value
+    ptrs = []
+    for align, nmemb, size in ((16, 10, 8), (32, 1000, 16), (256, 3, 1), (4096, 1, 100)):
+        ptr = L._spike_malloc(align, nmemb, size, b"test", 1)
+        assert ptr and ptr % max(align, 64) == 0
+        ctypes.memset(ptr, 0x5a, nmemb * size)           # the whole buffer is writable
+        ptrs.append(ptr)
+    assert cnt_a.value == a0 + 4
+    L.spike_free(ptrs[0])
+    L.spike_nullify(ptrs[1])
+    L.spike_free(ptrs[2])
+    L.spike_free(ptrs[3])
+    L.spike_free(None)                                    # ignored, still counted (spike_memory_host.c:42-50)
+    assert cnt_f.value == f0 + 5
+    assert not L._spike_malloc(16, 2 ** 62, 16, b"test", 2)   # overflow: NULL, no abort
+    assert cnt_a.value == a0 + 4

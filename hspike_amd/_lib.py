@@ -118,6 +118,7 @@ EXPORTED = [
     "spike_get_schedule", "spike_get_tips", "spike_get_reduced_solution", "spike_set_stream",
     "spike_error_string", "spike_last_error_message", "spike_batch_create", "spike_batch_destroy", "spike_batch_solve_dev",
     "spike_batch_get_stats", "spike_autotune_partitions", "spike_residual_host", "spike_band_solve_host",
+    "spike_band_solve_dev_noalloc",
     "spike_gen_banded",
     "spike_gen_dense", "spike_rcm_order", "spike_permute_csr",
     "spike_server_default_opts", "spike_server_create", "spike_server_destroy", "spike_server_submit",
@@ -160,6 +161,7 @@ def load():
         "spike_autotune_partitions": (i32, [i32, i32, i32, i32, i32, i32]),
         "spike_residual_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, vp, vp]),
         "spike_band_solve_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, vp]),
+        "spike_band_solve_dev_noalloc": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, vp, vp]),
         "spike_gen_banded": (i32, [i32, i32, i32, i32, dbl, dbl, dbl, dbl, dbl, ctypes.c_uint64, vp, vp, vp,
                                    ctypes.POINTER(i64)]),
         "spike_gen_dense": (i32, [i32, i64, dbl, dbl, ctypes.c_uint64, vp]),

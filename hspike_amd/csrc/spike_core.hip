@@ -1642,88 +1642,177 @@ int residual_dev(int dtype, int n, const int *d_rowptr, const int *d_colind, con
     return SPIKE_SUCCESS;
 }
 
-template <typename H>
-int axpy_dev(void *x, const void *d, size_t count, hipStream_t st) {
-    if (!count) return 0;
-    const unsigned grid = (unsigned)std::min<size_t>((count + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_add<H>, dim3(grid), dim3(256), 0, st, (H *)x, (const H *)d, (int64_t)count);
-    HIPCK(hipGetLastError());
-    return 0;
-}
-
-// Band solve of one matrix on the device: LU with partial pivoting of its band, then gbtrs on
-// the panel x (n x nrhs, leading dimension n) in place.
-struct BandOne {
-    DevBuf ab, piv, info, sysd, items;
-    BandSys B{};
-    size_t lds_f = 0;
+// ---------------------------------------------------------------- band solve on a workspace
+// The whole device workspace of one band solve is carved from one region in a fixed order, so
+// the allocating entry point (one device buffer per call) and the allocation-free one (a static
+// device array) run the same kernels on the same layout and produce the same bits.
+struct Carve {
+    char *base;
+    size_t used = 0, cap;
+    Carve(void *p, size_t c) : base((char *)p), cap(c) {}
+    template <typename X> X *take(size_t count) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        used = at + count * sizeof(X);
+        return (base && used <= cap) ? (X *)(base + at) : nullptr;
+    }
 };
 
-int band_factor_dev(int dtype, int n, int ku, int kl, const int *d_rowptr, const int *d_colind, const void *d_aij,
-                    BandOne &F, hipStream_t st) {
+struct BandWork {
+    int *flags;            // [0..2] ku, kl, malformed (k_bandwidth); [3] LU info; [4] CSR ends
+    RowChunk *chunks;      // ceil(n / RES_ROWS) row chunks (>= the ceil(n / BW_ROWS) of k_bandwidth)
+    SysDev *sysdev;
+    int *c0s;              // {0, chunks} for k_norm_reduce
+    double *partials, *norms;
+    BandSys *sys;
+    int *items;
+    int *piv;
+    void *ab, *r;
+};
+
+// Carves the workspace of an n x n band solve with nrhs columns; null members when it does not fit.
+BandWork band_carve(Carve &c, int dtype, int n, int ku, int kl, int nrhs) {
     const size_t hs = h_bytes(dtype);
-    F.B = BandSys{};
-    F.B.n = n; F.B.kl = kl; F.B.ku = ku; F.B.ldab = 2 * kl + ku + 1;
-    F.lds_f = (size_t)(std::max(kl, 1) + kl + ku + 1) * hs;
-    if (F.lds_f > 64 * 1024) {
+    const int nch = (n + RES_ROWS - 1) / RES_ROWS, ldab = 2 * kl + ku + 1;
+    BandWork w;
+    w.flags = c.take<int>(8);
+    w.chunks = c.take<RowChunk>(nch);
+    w.sysdev = c.take<SysDev>(1);
+    w.c0s = c.take<int>(2);
+    w.partials = c.take<double>((size_t)nch * 64 * 2);
+    w.norms = c.take<double>(128);
+    w.sys = c.take<BandSys>(1);
+    w.items = c.take<int>(3 * (size_t)(nrhs + 1));
+    w.piv = c.take<int>(n);
+    w.ab = c.take<char>((size_t)n * ldab * hs);
+    w.r = c.take<char>((size_t)n * std::max(nrhs, 1) * hs);
+    return w;
+}
+
+size_t band_ws_bytes(int dtype, int n, int ku, int kl, int nrhs) {
+    Carve c(nullptr, SIZE_MAX);
+    band_carve(c, dtype, n, ku, kl, nrhs);
+    return c.used;
+}
+
+// Column-major panels of H per column chunk of 64: r = b - A x and each column's relative
+// residual into rel (caller-sized, nrhs entries); w.chunks already hold the RES_ROWS chunks.
+int band_residual(int dtype, int n, const int *rowptr, const int *colind, const void *aij, int nrhs, const void *b, int ldb,
+                  const void *x, int ldx, void *r, const BandWork &w, double *rel, hipStream_t st) {
+    const size_t hs = h_bytes(dtype);
+    const int nch = (n + RES_ROWS - 1) / RES_ROWS;
+    for (int c0 = 0; c0 < nrhs; c0 += 64) {
+        const int m = std::min(64, nrhs - c0);
+        SysDev sd;
+        memset(&sd, 0, sizeof sd);
+        sd.rowptr = rowptr; sd.colind = colind; sd.aij = aij; sd.n = n; sd.m = m;
+        sd.ldb = ldb; sd.ldx = ldx; sd.ldr = n;
+        sd.bptr = (const char *)b + (size_t)c0 * ldb * hs;
+        sd.xptr = (char *)const_cast<void *>(x) + (size_t)c0 * ldx * hs;
+        sd.rptr = (char *)r + (size_t)c0 * n * hs;
+        hipLaunchKernelGGL(k_put<SysDev>, dim3(1), dim3(64), 0, st, w.sysdev, sd);
+        int e;
+        if ((e = launch_residual(dtype, m, 64, false, (unsigned)nch, st, w.chunks, w.sysdev, w.partials, (const int *)nullptr)))
+            return e;
+        hipLaunchKernelGGL(k_norm_reduce, dim3(64), dim3(256), 0, st, (const int *)w.c0s, (const SysDev *)w.sysdev,
+                           (const double *)w.partials, w.norms, 64);
+        HIPCK(hipGetLastError());
+        double nr[128];
+        HIPCK(hipMemcpyAsync(nr, w.norms, sizeof nr, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        for (int c = 0; c < m; ++c) rel[c0 + c] = nr[2 * c + 1] > 0 ? sqrt(nr[2 * c] / nr[2 * c + 1]) : sqrt(nr[2 * c]);
+    }
+    return SPIKE_SUCCESS;
+}
+
+// gbtrs on a panel p (n x nrhs, leading dimension ldp) in place
+int band_trs(int dtype, int n, int ku, int kl, int nrhs, void *p, int ldp, const BandWork &w, hipStream_t st) {
+    const size_t hs = h_bytes(dtype), per_col = (size_t)n * hs, cap = 48 * 1024;
+    const bool lx = per_col <= cap;
+    const int mc = lx ? (int)std::min<size_t>({64, (size_t)nrhs, cap / per_col}) : std::min(64, nrhs);
+    BandSys B{};
+    B.n = n; B.kl = kl; B.ku = ku; B.ldab = 2 * kl + ku + 1; B.m = nrhs; B.ldx = ldp; B.ab = 0; B.piv = 0; B.x = 0;
+    const int ni = (nrhs + mc - 1) / mc;
+    hipLaunchKernelGGL(k_put<BandSys>, dim3(1), dim3(64), 0, st, w.sys, B);
+    hipLaunchKernelGGL(k_band_items, dim3((ni + 255) / 256), dim3(256), 0, st, w.items, nrhs, mc);
+    const size_t lds = lx ? per_col * mc : 0;
+    if (dtype == SPIKE_DTYPE_Z) {
+        if (lx) hipLaunchKernelGGL((k_gbtrs<zd, true>), dim3(ni), dim3(256), lds, st, (const BandSys *)w.sys, (const int *)w.items,
+                                   (const zd *)w.ab, (const int *)w.piv, (zd *)p);
+        else hipLaunchKernelGGL((k_gbtrs<zd, false>), dim3(ni), dim3(256), 0, st, (const BandSys *)w.sys, (const int *)w.items,
+                                (const zd *)w.ab, (const int *)w.piv, (zd *)p);
+    } else {
+        if (lx) hipLaunchKernelGGL((k_gbtrs<double, true>), dim3(ni), dim3(256), lds, st, (const BandSys *)w.sys,
+                                   (const int *)w.items, (const double *)w.ab, (const int *)w.piv, (double *)p);
+        else hipLaunchKernelGGL((k_gbtrs<double, false>), dim3(ni), dim3(256), 0, st, (const BandSys *)w.sys,
+                                (const int *)w.items, (const double *)w.ab, (const int *)w.piv, (double *)p);
+    }
+    HIPCK(hipGetLastError());
+    return SPIKE_SUCCESS;
+}
+
+// Band LU with partial pivoting of the device CSR (bandwidths ku, kl), x = A^{-1} b, then FP64
+// refinement x += A^{-1}(b - A x) while the largest relative residual halves (at most three
+// corrections, the reference's PARDISO setting for R, spike_algebra.c:299).  rel receives every
+// column's final relative residual (nrhs entries, caller memory).  All device memory is `w`.
+int band_run(int dtype, int n, int ku, int kl, const int *rowptr, const int *colind, const void *aij, int nrhs,
+             const void *b, int ldb, void *x, int ldx, double *rel, const BandWork &w, hipStream_t st) {
+    const size_t hs = h_bytes(dtype);
+    const int ldab = 2 * kl + ku + 1, nch = (n + RES_ROWS - 1) / RES_ROWS;
+    const size_t lds_f = (size_t)(std::max(kl, 1) + kl + ku + 1) * hs;
+    if (lds_f > 64 * 1024) {
         set_err("band too wide for the device band LU (kl %ld, ku %ld)", (long)kl, (long)ku);
         return SPIKE_ERR_UNSUPPORTED;
     }
-    const size_t elems = (size_t)n * F.B.ldab;
-    int e;
-    if ((e = F.ab.ensure(elems * hs)) || (e = F.piv.ensure((size_t)n * sizeof(int))) || (e = F.info.ensure(sizeof(int))) ||
-        (e = upload(F.sysd, &F.B, sizeof F.B, st)))
-        return e;
-    const unsigned rg = (unsigned)((n + 255) / 256);
+    BandSys B{};
+    B.n = n; B.kl = kl; B.ku = ku; B.ldab = ldab;
+    const size_t elems = (size_t)n * ldab;
+    const unsigned rg = (unsigned)((n + 255) / 256), fg = (unsigned)std::min<size_t>((elems + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_put<BandSys>, dim3(1), dim3(64), 0, st, w.sys, B);
+    hipLaunchKernelGGL(k_make_chunks, dim3((nch + 255) / 256), dim3(256), 0, st, w.chunks, nch, RES_ROWS);
+    hipLaunchKernelGGL(k_put<int>, dim3(1), dim3(64), 0, st, w.c0s, 0);
+    hipLaunchKernelGGL(k_put<int>, dim3(1), dim3(64), 0, st, w.c0s + 1, nch);
     if (dtype == SPIKE_DTYPE_Z) {
-        if ((e = fill_zero<zd>(F.ab, elems, st))) return e;
-        hipLaunchKernelGGL(k_csr_to_band<zd>, dim3(rg), dim3(256), 0, st, d_rowptr, d_colind, (const zd *)d_aij, n, kl, ku,
-                           F.B.ldab, F.ab.as<zd>());
-        hipLaunchKernelGGL(k_gbtrf<zd>, dim3(1), dim3(GB_THREADS), F.lds_f, st, F.sysd.as<BandSys>(), F.ab.as<zd>(),
-                           F.piv.as<int>(), F.info.as<int>());
+        hipLaunchKernelGGL(k_fill<zd>, dim3(fg), dim3(256), 0, st, (zd *)w.ab, (int64_t)elems, s_zero_host<zd>());
+        hipLaunchKernelGGL(k_csr_to_band<zd>, dim3(rg), dim3(256), 0, st, rowptr, colind, (const zd *)aij, n, kl, ku, ldab,
+                           (zd *)w.ab);
+        hipLaunchKernelGGL(k_gbtrf<zd>, dim3(1), dim3(GB_THREADS), lds_f, st, (const BandSys *)w.sys, (zd *)w.ab, w.piv,
+                           w.flags + 3);
     } else {
-        if ((e = fill_zero<double>(F.ab, elems, st))) return e;
-        hipLaunchKernelGGL(k_csr_to_band<double>, dim3(rg), dim3(256), 0, st, d_rowptr, d_colind, (const double *)d_aij, n,
-                           kl, ku, F.B.ldab, F.ab.as<double>());
-        hipLaunchKernelGGL(k_gbtrf<double>, dim3(1), dim3(GB_THREADS), F.lds_f, st, F.sysd.as<BandSys>(), F.ab.as<double>(),
-                           F.piv.as<int>(), F.info.as<int>());
+        hipLaunchKernelGGL(k_fill<double>, dim3(fg), dim3(256), 0, st, (double *)w.ab, (int64_t)elems, 0.0);
+        hipLaunchKernelGGL(k_csr_to_band<double>, dim3(rg), dim3(256), 0, st, rowptr, colind, (const double *)aij, n, kl, ku,
+                           ldab, (double *)w.ab);
+        hipLaunchKernelGGL(k_gbtrf<double>, dim3(1), dim3(GB_THREADS), lds_f, st, (const BandSys *)w.sys, (double *)w.ab,
+                           w.piv, w.flags + 3);
     }
     HIPCK(hipGetLastError());
     int info = 0;
-    HIPCK(hipMemcpyAsync(&info, F.info.p, sizeof info, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(&info, w.flags + 3, sizeof info, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     if (info) {
         set_err("singular matrix (no nonzero pivot in column %ld)", (long)(info - 1));
         return SPIKE_ERR_SINGULAR;
     }
-    return SPIKE_SUCCESS;
-}
-
-int band_solve_dev(int dtype, BandOne &F, int nrhs, void *d_x, hipStream_t st) {
-    const size_t hs = h_bytes(dtype);
-    const int n = F.B.n;
-    F.B.m = nrhs; F.B.ldx = n; F.B.x = 0;
-    const size_t per_col = (size_t)n * hs, cap = 48 * 1024;
-    const bool lx = per_col <= cap;
-    const int mc = lx ? (int)std::min<size_t>({64, (size_t)nrhs, cap / per_col}) : std::min(64, nrhs);
-    std::vector<int> items;
-    for (int c = 0; c < nrhs; c += mc) items.insert(items.end(), {0, c, std::min(mc, nrhs - c)});
-    int e;
-    if ((e = upload(F.sysd, &F.B, sizeof F.B, st)) || (e = upload(F.items, items.data(), items.size() * sizeof(int), st))) return e;
-    const unsigned ni = (unsigned)(items.size() / 3);
-    const size_t lds = lx ? per_col * mc : 0;
-    if (dtype == SPIKE_DTYPE_Z) {
-        if (lx) hipLaunchKernelGGL((k_gbtrs<zd, true>), dim3(ni), dim3(256), lds, st, F.sysd.as<BandSys>(), F.items.as<int>(),
-                                   (const zd *)F.ab.p, (const int *)F.piv.p, (zd *)d_x);
-        else hipLaunchKernelGGL((k_gbtrs<zd, false>), dim3(ni), dim3(256), 0, st, F.sysd.as<BandSys>(), F.items.as<int>(),
-                                (const zd *)F.ab.p, (const int *)F.piv.p, (zd *)d_x);
-    } else {
-        if (lx) hipLaunchKernelGGL((k_gbtrs<double, true>), dim3(ni), dim3(256), lds, st, F.sysd.as<BandSys>(),
-                                   F.items.as<int>(), (const double *)F.ab.p, (const int *)F.piv.p, (double *)d_x);
-        else hipLaunchKernelGGL((k_gbtrs<double, false>), dim3(ni), dim3(256), 0, st, F.sysd.as<BandSys>(), F.items.as<int>(),
-                                (const double *)F.ab.p, (const int *)F.piv.p, (double *)d_x);
-    }
+    if (nrhs == 0) return SPIKE_SUCCESS;
+    const unsigned pg = (unsigned)std::min<int64_t>(((int64_t)n * nrhs + 255) / 256, 65536);
+    if (dtype == SPIKE_DTYPE_Z) hipLaunchKernelGGL(k_copy2d<zd>, dim3(pg), dim3(256), 0, st, (zd *)x, ldx, (const zd *)b, ldb, n, nrhs);
+    else hipLaunchKernelGGL(k_copy2d<double>, dim3(pg), dim3(256), 0, st, (double *)x, ldx, (const double *)b, ldb, n, nrhs);
     HIPCK(hipGetLastError());
+    int e;
+    if ((e = band_trs(dtype, n, ku, kl, nrhs, x, ldx, w, st))) return e;
+    if ((e = band_residual(dtype, n, rowptr, colind, aij, nrhs, b, ldb, x, ldx, w.r, w, rel, st))) return e;
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c) worst = std::max(worst, rel[c]);
+    for (int it = 0; it < 3 && worst > 0.0 && std::isfinite(worst); ++it) {
+        if ((e = band_trs(dtype, n, ku, kl, nrhs, w.r, n, w, st))) return e;          // delta = A^{-1} r, in place
+        if (dtype == SPIKE_DTYPE_Z) hipLaunchKernelGGL(k_add2d<zd>, dim3(pg), dim3(256), 0, st, (zd *)x, ldx, (const zd *)w.r, n, n, nrhs);
+        else hipLaunchKernelGGL(k_add2d<double>, dim3(pg), dim3(256), 0, st, (double *)x, ldx, (const double *)w.r, n, n, nrhs);
+        HIPCK(hipGetLastError());
+        if ((e = band_residual(dtype, n, rowptr, colind, aij, nrhs, b, ldb, x, ldx, w.r, w, rel, st))) return e;
+        double wn = 0.0;
+        for (int c = 0; c < nrhs; ++c) wn = std::max(wn, rel[c]);
+        if (!(wn < 0.5 * worst)) break;               // stagnating: the correction is at rounding level
+        worst = wn;
+    }
     return SPIKE_SUCCESS;
 }
 
@@ -1753,9 +1842,8 @@ int spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const 
     return residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rel, ws, 0);
 }
 
-// Direct solve of a banded CSR matrix by the device band LU with partial pivoting, refined in
-// FP64 (x += A^{-1} (b - A x)) while the largest relative residual keeps shrinking (up to three
-// corrections, the reference's PARDISO setting for R, spike_algebra.c:299).
+// Direct solve of a banded CSR matrix (host arrays) by the device band LU with partial pivoting
+// and FP64 refinement (band_run); the workspace is one device buffer of this call.
 int spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind, const void *aij, int nrhs,
                           const void *b, int ldb, void *x, int ldx, double *rel) {
     if (n <= 0 || nnz < 0 || nnz > INT32_MAX || nrhs < 0 || !rowptr || (nnz > 0 && (!colind || !aij)) ||
@@ -1766,45 +1854,113 @@ int spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, cons
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { (void)hipGetLastError(); set_err("no HIP device%s", ""); return SPIKE_ERR_DEVICE; }
     integer_t ku = 0, kl = 0;
     matrix_ComputeBandwidth(n, colind, rowptr, nullptr, &ku, &kl);
-    const size_t hs = h_bytes(dtype);
-    DevBuf drp, dci, da, dx, db, dr;
+    const size_t hs = h_bytes(dtype), panel = (size_t)n * nrhs * hs;
+    DevBuf drp, dci, da, dx, db, ws;
     int e;
     if ((e = drp.ensure((size_t)(n + 1) * 4)) || (e = dci.ensure((size_t)std::max<int64_t>(nnz, 1) * 4)) ||
-        (e = da.ensure((size_t)std::max<int64_t>(nnz, 1) * hs)))
+        (e = da.ensure((size_t)std::max<int64_t>(nnz, 1) * hs)) || (e = dx.ensure(panel)) || (e = db.ensure(panel)) ||
+        (e = ws.ensure(band_ws_bytes(dtype, n, ku, kl, nrhs))))
         return e;
     HIPCK(hipMemcpy(drp.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
     if (nnz) {
         HIPCK(hipMemcpy(dci.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(da.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
     }
-    BandOne F;
-    if ((e = band_factor_dev(dtype, n, ku, kl, drp.as<int>(), dci.as<int>(), da.p, F, 0))) return e;
-    if (nrhs == 0) return SPIKE_SUCCESS;
-    const size_t panel = (size_t)n * nrhs * hs;
-    if ((e = dx.ensure(panel)) || (e = db.ensure(panel)) || (e = dr.ensure(panel))) return e;
-    HIPCK(hipMemcpy2D(db.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(dx.p, db.p, panel, hipMemcpyDeviceToDevice));
-    if ((e = band_solve_dev(dtype, F, nrhs, dx.p, 0))) return e;
-    std::vector<double> rl(nrhs);
-    ResidualWs ws;
-    if ((e = residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rl.data(), ws, 0))) return e;
-    double worst = *std::max_element(rl.begin(), rl.end());
-    for (int it = 0; it < 3 && worst > 0.0 && std::isfinite(worst); ++it) {
-        if ((e = band_solve_dev(dtype, F, nrhs, dr.p, 0))) return e;          // delta = A^{-1} r, in place
-        if (dtype == SPIKE_DTYPE_Z) { if ((e = axpy_dev<zd>(dx.p, dr.p, (size_t)n * nrhs, 0))) return e; }
-        else if ((e = axpy_dev<double>(dx.p, dr.p, (size_t)n * nrhs, 0))) return e;
-        std::vector<double> rn(nrhs);
-        if ((e = residual_dev(dtype, n, drp.as<int>(), dci.as<int>(), da.p, nrhs, dx.p, db.p, dr.p, rn.data(), ws, 0))) return e;
-        const double wn = *std::max_element(rn.begin(), rn.end());
-        rl = rn;
-        if (!(wn < 0.5 * worst)) break;               // stagnating: the correction is at rounding level
-        worst = wn;
-    }
-    HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, dx.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
-    if (rel) for (int c = 0; c < nrhs; ++c) rel[c] = rl[c];
+    if (nrhs) HIPCK(hipMemcpy2D(db.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
+    Carve c(ws.p, ws.bytes);
+    const BandWork w = band_carve(c, dtype, n, ku, kl, nrhs);
+    std::vector<double> rl(std::max(nrhs, 1));
+    if ((e = band_run(dtype, n, ku, kl, drp.as<int>(), dci.as<int>(), da.p, nrhs, db.p, n, dx.p, n, rl.data(), w, 0))) return e;
+    if (nrhs) HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, dx.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
+    if (rel) for (int q = 0; q < nrhs; ++q) rel[q] = rl[q];
     return SPIKE_SUCCESS;
 }
 
 }  // extern "C"
+
+// ============================================================================ allocation-free band solve
+// Workspace of spike_band_solve_dev_noalloc: a static device array, so the entry point calls no
+// allocator.  Each GPU the library runs on holds its own copy until the process ends.  In host
+// code the array's name is not a device address: the address on the current device comes from
+// hipGetSymbolAddress, resolved once per device and kept in scratch_dev[].
+static __device__ __align__(256) unsigned char scratch[SPIKE_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_dev[SCRATCH_MAX_DEVICES];
+
+int scratch_on_current_device(void **out) {
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        set_err("device ordinal %ld beyond the scratch table", (long)dev);
+        return SPIKE_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_err("no device address for the scratch area (%s)", hipGetErrorString(e));
+            return SPIKE_ERR_DEVICE;
+        }
+        scratch_dev[dev] = p;
+    }
+    *out = scratch_dev[dev];
+    return SPIKE_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int spike_band_solve_dev_noalloc(int dtype, int n, int64_t nnz, const int *d_rowptr, const int *d_colind,
+                                            const void *d_aij, int nrhs, const void *d_b, int ldb, void *d_x, int ldx,
+                                            double *rel, void *stream) {
+    if (n <= 0 || nnz < 0 || nnz > INT32_MAX || nrhs < 0 || !d_rowptr || (nnz > 0 && (!d_colind || !d_aij)) ||
+        (nrhs > 0 && (!d_b || !d_x || !rel)) || ldb < n || ldx < n || (dtype != SPIKE_DTYPE_D && dtype != SPIKE_DTYPE_Z))
+        return SPIKE_ERR_INVALID_ARG;
+    void *base = nullptr;
+    int e = scratch_on_current_device(&base);
+    if (e) return e;
+    hipStream_t st = (hipStream_t)stream;
+    // CSR checks and bandwidths on the device, from the head of the area: flags, row chunks, view
+    const int nbw = (n + BW_ROWS - 1) / BW_ROWS;
+    Carve head(base, SPIKE_NOALLOC_SCRATCH_BYTES);
+    int *flags = head.take<int>(8);
+    RowChunk *ch = head.take<RowChunk>(nbw);
+    SysDev *sd = head.take<SysDev>(1);
+    if (!sd) {
+        set_err("n = %ld needs more than the %ld-byte scratch area", (long)n, (long)SPIKE_NOALLOC_SCRATCH_BYTES);
+        return SPIKE_ERR_OOM;
+    }
+    SysDev sv;
+    memset(&sv, 0, sizeof sv);
+    sv.rowptr = d_rowptr; sv.colind = d_colind; sv.n = n;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, st, flags, (int64_t)8, 0);   // the area keeps the last call's bytes
+    hipLaunchKernelGGL(k_make_chunks, dim3((nbw + 255) / 256), dim3(256), 0, st, ch, nbw, BW_ROWS);
+    hipLaunchKernelGGL(k_put<SysDev>, dim3(1), dim3(64), 0, st, sd, sv);
+    hipLaunchKernelGGL(k_bandwidth, dim3(nbw), dim3(256), 0, st, (const RowChunk *)ch, (const SysDev *)sd, flags);
+    hipLaunchKernelGGL(k_csr_ends, dim3(1), dim3(64), 0, st, d_rowptr, n, nnz, flags + 4);
+    HIPCK(hipGetLastError());
+    int fl[8];
+    HIPCK(hipMemcpyAsync(fl, flags, sizeof fl, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (fl[2] || fl[4]) {
+        set_err("malformed CSR%s", "");
+        return SPIKE_ERR_MALFORMED_CSR;
+    }
+    const int ku = fl[0], kl = fl[1];
+    const size_t need = band_ws_bytes(dtype, n, ku, kl, nrhs);
+    if (need > SPIKE_NOALLOC_SCRATCH_BYTES) {
+        set_err("this solve needs %ld bytes of workspace; the allocation-free entry point has %ld", (long)need,
+                (long)SPIKE_NOALLOC_SCRATCH_BYTES);
+        return SPIKE_ERR_OOM;
+    }
+    Carve c(base, SPIKE_NOALLOC_SCRATCH_BYTES);
+    const BandWork w = band_carve(c, dtype, n, ku, kl, nrhs);
+    return band_run(dtype, n, ku, kl, d_rowptr, d_colind, d_aij, nrhs, d_b, ldb, d_x, ldx, rel, w, st);
+}
 
 #include "spike_server.inc"

@@ -173,6 +173,21 @@ int  spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, con
                            const void *aij, int nrhs, const void *b, int ldb, void *x, int ldx,
                            double *rel);
 
+/* Allocation-free variant of spike_band_solve_host.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same computation and result bits as spike_band_solve_host,
+ * on device arrays (CSR, b with leading dimension ldb, x with ldx) and on `stream` (a
+ * hipStream_t, NULL = default); rel (host, nrhs entries) is required.  All workspace -- band,
+ * pivots, residual panel, descriptors -- lives in a static device area of
+ * SPIKE_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a solve that needs more
+ * (about n * (2 kl + ku + 1 + nrhs) values plus small descriptors) returns SPIKE_ERR_OOM.  The
+ * area keeps the last call's contents and is shared by every call on that GPU: calls that
+ * overlap on one GPU race and must be serialized by the caller. */
+#define SPIKE_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int  spike_band_solve_dev_noalloc(int dtype, int n, int64_t nnz, const int *d_rowptr,
+                                  const int *d_colind, const void *d_aij, int nrhs, const void *d_b,
+                                  int ldb, void *d_x, int ldx, double *rel, void *stream);
+
 /* ---------------- request server: continuous batching, memory admission, resident factors --
  * Requests (host arrays, kept alive and unmodified by the caller until spike_server_wait returns)
  * are admitted as worker slots and device memory free up; new matrices of one dtype are factored

@@ -54,9 +54,10 @@ C5 = dict(nreq=120, rate=40.0, frac_complex=0.2, n_lo=1e4, n_hi=5e6, k_lo=2, k_h
           nrhs_hi=64, repeat=0.3, fill=0.30, dom=1.2, tol=1e-10, seed=20261205)
 
 
-def c5_stream(nreq, seed):
+def c5_stream(nreq, seed, n_hi=None):
     """The C5 request stream: (arrival offset s, matrix id, nrhs, RHS seed) per request and
-    (dtype complex?, n, k, matrix seed) per distinct matrix, drawn from `seed`."""
+    (dtype complex?, n, k, matrix seed) per distinct matrix, drawn from `seed`; n_hi caps the
+    row-count range (the tests draw the same stream shape at sizes the oracle solves quickly)."""
     rng = np.random.RandomState(seed)
     t = np.cumsum(rng.exponential(1.0, nreq))
     mats, reqs = [], []
@@ -65,7 +66,7 @@ def c5_stream(nreq, seed):
             mid = int(rng.randint(len(mats)))
         else:
             cplx = rng.rand() < C5["frac_complex"]
-            n = int(np.exp(rng.uniform(np.log(C5["n_lo"]), np.log(C5["n_hi"]))))
+            n = int(np.exp(rng.uniform(np.log(C5["n_lo"]), np.log(n_hi or C5["n_hi"]))))
             k = int(rng.randint(C5["k_lo"], (C5["k_hi_complex"] if cplx else C5["k_hi"]) + 1))
             mats.append((cplx, n, k, seed * 131 + len(mats)))
             mid = len(mats) - 1

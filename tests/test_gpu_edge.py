@@ -78,7 +78,9 @@ def test_degenerate_and_extreme_shapes(hs, oracle_mod, n, ku, kl, P, prec, nrhs)
 @pytest.mark.parametrize("rel", ["heptadiagonal/small.bin", "heptadiagonal/medium.bin", "pentadiagonal/small.bin",
                                  "pentadiagonal/large.bin", "dummy/tridiagonal.bin", "bandwidth/banded.bin"])
 def test_reference_fixture_matrices(hs, oracle_mod, rel):
-    """The reference's test matrices, solved as its driver does (main.c:103-111: B = ones, P = 4)."""
+    """The reference's test matrices through dspike_core_host with B = ones and a random column
+    (main.c:103-111); the partition count is autotuned here (one partition for these small n),
+    the reference's own P = 4 and P = 2-3 are covered in test_gpu_direct.py."""
     Ao = oracle_mod.load_csr(ref_path(rel))
     A = to_h(hs, Ao)
     B = np.ones((A.n, 2))

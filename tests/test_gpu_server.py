@@ -1,7 +1,8 @@
 """Request server (spike_server_*): continuous batching with overlapping jobs, memory admission
 and resident factors.  Each result must equal -- bit for bit, the stated tolerance is zero --
 the solution of the same request solved alone through a fresh handle, whatever else was in
-flight; every column meets the 1e-10 relative-residual bar."""
+flight; every column meets the 1e-10 relative-residual bar, and every request agrees with the
+oracle's FP64 band LU within 1e-9 (relative)."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,7 @@ def alone(hs, A, B, prec):
 
 
 @pytest.mark.parametrize("prec", ["mixed", "fp64"])
-def test_mix_equals_alone(hs, prec):
+def test_mix_equals_alone(hs, oracle_mod, prec):
     mats = make_mats(hs, 3, 8) if prec == "mixed" else [m for m in make_mats(hs, 4, 8) if m.bandwidth()[0] <= 64 or m.dtype_code == 0]
     rng = np.random.RandomState(11)
     srv = hs.Server(prec, tol=1e-10, workers=3, max_batch=4)
@@ -58,6 +59,8 @@ def test_mix_equals_alone(hs, prec):
     assert hits >= 1 and st["p99_ms"] >= st["p50_ms"] > 0
     for A, B, X in out:
         assert hs.residual(A, X, B).max() <= 1e-10
+        Xo = oracle_mod.direct_solve(oracle_mod.CSR(A.n, A.rowptr, A.colind, A.aij), B)
+        assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 1e-9
         np.testing.assert_array_equal(X, alone(hs, A, B, prec))
 
 

@@ -7,20 +7,28 @@ resident in HBM: CSR -> blocks, block LU, spike tips, reduced system, recovery a
 residual / refinement loop, ending when every column's relative residual meets the tolerance.
 
 Workloads (synthetic, seeded):
-  c2 (default) n = 10^6, half-bandwidth 64, 15% in-band fill U[-1,1], diagonal 1.05*sum|offdiag|
+  c4 (default) 4096 independent systems in one batched solve: n log-uniform [10^3, 2*10^5],
+               half-bandwidth uniform in {4, 8, 16, 32, 64}, 30% fill, dominance 1.1, nrhs 1..8
+  c2           n = 10^6, half-bandwidth 64, 15% in-band fill U[-1,1], diagonal 1.05*sum|offdiag|
                (floor 1), 16 RHS U[-1,1]; plain FP64 is also run at the configured 32 partitions
   c3           n = 2*10^6, half-bandwidth 128, 20% fill, per-row dominance U[1.01, 2.0], 64 RHS,
                refinement to 1e-12
   c1           n = 10^4 pentadiagonal, a_ii = sum|a_ij| + 1, 4 partitions, 1 RHS
-  c4           4096 independent systems in one batched solve: n log-uniform [10^3, 2*10^5],
-               half-bandwidth uniform in {4, 8, 16, 32, 64}, 30% fill, dominance 1.1, nrhs 1..8
   c5           request serving: Poisson arrivals (rate --rate), 80% real / 20% complex128, n
                log-uniform [10^4, 5*10^6], half-bandwidth 2..128 (complex 2..64), 30% fill,
                dominance 1.2, nrhs 1..64, 30% of requests repeat a resident matrix with a new RHS
 The CPU baseline is the oracle (oracle/, a C restatement of the reference's SPIKE flow with a
-banded partial-pivot LU in place of SuperLU_MT) timed on the host cores on the same matrix.
+banded partial-pivot LU in place of SuperLU_MT), timed on the host cores on a bounded sample of
+the same workload: on every core at once (independent solves spread over the cores, each solve
+on one core -- the reference's driver loops over its partitions serially,
+OpenMP/spike_interfaces.c:105-237), and one solve after another on one core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1] [--no-cpu]
+Every stage is reported against both peaks (`stage_roofline`); `roofline` is the stage with the
+largest time per step.  Per-launch times are the library's HIP events on the stream it launches
+on; `traffic` (HBM bytes per launch) and MFMA busy cycles come from the committed rocprofv3
+counter summaries in profiles/ for this workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3|c1|c5] [--no-cpu]
 """
 import argparse
 import json
@@ -48,10 +56,17 @@ CONFIGS = {
 SEED_A, SEED_B = 20261202, 7
 C4 = dict(nsys=4096, n_lo=1e3, n_hi=2e5, ks=(4, 8, 16, 32, 64), fill=0.30, dom=1.1, nrhs_hi=8, tol=1e-10,
           seed=20261204)
-
-
 C5 = dict(nreq=120, rate=40.0, frac_complex=0.2, n_lo=1e4, n_hi=5e6, k_lo=2, k_hi=128, k_hi_complex=64,
           nrhs_hi=64, repeat=0.3, fill=0.30, dom=1.2, tol=1e-10, seed=20261205)
+
+# stage -> stats key of its time (the library's HIP events around the stage's launches)
+STAGES = {"factor": "ms_factor", "spikes": "ms_spikes", "ysweep": "ms_ysweep", "recover": "ms_recover",
+          "residual": "ms_residual"}
+PER_PASS = ("ysweep", "recover", "residual")          # launched once per solve / refinement pass
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 def c5_stream(nreq, seed, n_hi=None):
@@ -84,9 +99,9 @@ def c4_systems(nsys, seed):
 
 
 def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
-    """Algorithmic flops and HBM bytes of one launch of a stage: the counting formulas of the
-    benchmark definition (band read/written once, RHS and solution panels once per pass);
-    implementation overheads never enter."""
+    """Algorithmic flops and HBM bytes of one system's share of one launch of a stage: the
+    counting formulas of the benchmark definition (band read/written once, RHS and solution
+    panels once per pass); implementation overheads never enter."""
     if name == "factor":      # banded LU of all partitions
         return 2.0 * n * kl * ku, 2.0 * n * (kl + ku + 1) * s
     if name == "spikes":      # V/W spike tips: m_i = ku [i < P-1] + kl [i > 0] columns per partition
@@ -101,25 +116,148 @@ def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
     raise KeyError(name)
 
 
-def measured_traffic(config, kernel):
-    """HBM bytes per dispatch of `kernel` on `config` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-calibrated); None when that kernel/workload was not measured."""
-    path = os.path.join(ROOT, "profiles", "spike_%s_mixed_hbm_traffic.md" % config)
+def profile_table(config, kind):
+    """Per-kernel figures from the committed counter summaries of `config`:
+    kind 'traffic' -> HBM bytes per dispatch (profiles/spike_<config>_mixed_hbm_traffic.md),
+    kind 'mfma' -> MFMA busy fraction (profiles/spike_<config>_mixed_mfma.md)."""
+    name = {"traffic": "spike_%s_mixed_hbm_traffic.md", "mfma": "spike_%s_mixed_mfma.md"}[kind] % config
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
+    key = "traffic_bytes" if kind == "traffic" else "mfma_busy"
     for line in open(path):
         parts = line.split()
-        if len(parts) == 5 and parts[0] == "traffic_bytes" and parts[1] == config and parts[2] == kernel:
-            return int(parts[4])
-    return None
+        if len(parts) == 5 and parts[0] == key and parts[1] == config:
+            out[parts[2]] = float(parts[4])
+    return out
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "cpu_model": model}
+
+
+def oracle_baseline(jobs, unit_per_job, budget_s, serial_budget_s, what):
+    """Time the oracle on a bounded prefix of `jobs`, each a pair (prepare() -> data, solve(data,
+    nthreads)) whose preparation (input generation) stays outside the timed region: first on every
+    host core at once (one solve per core, a pool of threads; the oracle releases the interpreter
+    lock), then one after another on one core; returns the cpu_baseline record."""
+    from concurrent.futures import ThreadPoolExecutor
+    info = cpu_info()
+    ncores = info["nproc"]
+    t_par = 0.0
+    done_units = done_jobs = 0
+    with ThreadPoolExecutor(max_workers=ncores) as ex:
+        idx = 0
+        while idx < len(jobs) and t_par < budget_s:
+            # batches of several solves per core, scheduled dynamically, so that the uneven
+            # solve sizes leave few cores idle at the end of a batch
+            batch = list(range(idx, min(len(jobs), idx + 4 * ncores)))
+            data = list(ex.map(lambda i: jobs[i][0](), batch))
+            t0 = time.perf_counter()
+            list(ex.map(lambda q: jobs[batch[q]][1](data[q], 1), range(len(batch))))
+            t_par += time.perf_counter() - t0
+            done_units += sum(unit_per_job[i] for i in batch)
+            done_jobs += len(batch)
+            idx += len(batch)
+    t_ser = 0.0
+    s_units = s_jobs = 0
+    while s_jobs < len(jobs) and t_ser < serial_budget_s:
+        d = jobs[s_jobs][0]()
+        t0 = time.perf_counter()
+        jobs[s_jobs][1](d, 1)
+        t_ser += time.perf_counter() - t0
+        s_units += unit_per_job[s_jobs]
+        s_jobs += 1
+    log("cpu oracle: %d solves on %d cores in %.1f s; %d one after another in %.1f s" %
+        (done_jobs, ncores, t_par, s_jobs, t_ser))
+    return {"value": round(done_units / t_par, 3), "unit": "RHS/s", "cores": ncores, "kind": "port",
+            "sample": "%s: the first %d solves on all %d host cores (%.1f s); the first %d one after another on "
+                      "one core (%.1f s)" % (what, done_jobs, ncores, t_par, s_jobs, t_ser),
+            "serial_value": round(s_units / t_ser, 3), "serial_cores": 1, **info}
+
+
+def stage_roofline(avg_ms, counts, passes, groups, config, kernel_names):
+    """Every stage against HBM and the FP32 (factor stages) / FP64 (residual) peaks, per launch."""
+    traffic = profile_table(config, "traffic")
+    mfma = profile_table(config, "mfma")
+    out = {}
+    for st_name, st_ms in avg_ms.items():
+        if st_ms <= 0:
+            continue
+        reps = passes if st_name in PER_PASS else 1
+        nl = groups * reps
+        F, B = counts[st_name]
+        F, B = F * reps / nl, B * reps / nl          # per launch (the batch's systems split over groups)
+        t = st_ms / nl * 1e-3
+        pk = PEAK_TFLOPS["f64"] if st_name == "residual" else PEAK_TFLOPS["f32"]
+        kn = kernel_names[st_name]
+        rec = {"launches": nl, "ms_per_launch": round(st_ms / nl, 4), "kernel": kn,
+               "algorithmic_bytes": B, "algorithmic_flops": F,
+               "GB_s": round(B / t / 1e9, 2), "hbm_frac": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
+               "TFLOP_s": round(F / t / 1e12, 3), "mfma_frac": round(F / t / 1e12 / pk, 4),
+               "bound": "mfma" if F / B >= pk * 1e12 / (HBM_PEAK_GBS * 1e9) else "hbm"}
+        if kn in traffic:
+            rec["traffic"] = int(traffic[kn])
+            rec["traffic_over_algorithmic"] = round(traffic[kn] / B, 3)
+        if kn in mfma:
+            rec["mfma_busy"] = mfma[kn]
+        out[st_name] = rec
+    return out
+
+
+def roofline_of(stage_roof, avg_ms):
+    """The stage with the largest time per step: achieved algorithmic rate per launch on the side
+    its intensity puts it."""
+    dom = max(avg_ms, key=avg_ms.get)
+    r = stage_roof[dom]
+    if r["bound"] == "mfma":
+        pk = PEAK_TFLOPS["f64"] if dom == "residual" else PEAK_TFLOPS["f32"]
+        roof = {"bound": "mfma", "achieved": r["TFLOP_s"], "peak": pk, "unit": "TFLOP/s", "frac": r["mfma_frac"]}
+    else:
+        roof = {"bound": "hbm", "achieved": r["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["hbm_frac"]}
+    roof.update({"traffic": r.get("traffic"), "kernel": r["kernel"], "stage": dom, "launches_per_step": r["launches"],
+                 "algorithmic_flops": r["algorithmic_flops"], "algorithmic_bytes": r["algorithmic_bytes"],
+                 "ms_per_launch": r["ms_per_launch"]})
+    if "mfma_busy" in r:
+        roof["mfma_busy"] = r["mfma_busy"]
+    return roof
+
+
+def kernel_names_for(nts, nrhs_max):
+    """Instantiations the library launches for block sizes 16*NT (FP32 factors) and up to nrhs_max
+    RHS columns per chunk (column tiles per sweep workgroup CT with CT * NT <= 16); several block
+    sizes are joined with '+'."""
+    m = min(nrhs_max, 64)
+    ct_vw = lambda nt: min(4, 16 // nt)
+    ct_y = lambda nt: min(ct_vw(nt), 1 if m <= 16 else (2 if m <= 32 else 4))
+    need = (m + 1) // 2
+    tw = 4 if need <= 4 else (8 if need <= 8 else 16)
+    return {"factor": "+".join("k_factor<float,%d>" % nt for nt in nts),
+            "spikes": "+".join("k_sweep<float,%d,%d,true>" % (nt, ct_vw(nt)) for nt in nts),
+            "ysweep": "+".join("k_sweep<float,%d,%d,false>" % (nt, ct_y(nt)) for nt in nts),
+            "recover": "+".join("k_recover<float,%d,%d>" % (nt, ct_y(nt)) for nt in nts),
+            "residual": "k_residual<double,%d,2>" % tw}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c5"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS) + ["c4", "c5"])
     ap.add_argument("--nsys", type=int, default=C4["nsys"], help="c4: number of systems in the batch")
     ap.add_argument("--nreq", type=int, default=C5["nreq"], help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=C5["rate"], help="c5: Poisson arrival rate (requests/s)")
@@ -135,7 +273,7 @@ def main():
         for _ in range(3):
             torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
-        print("[bench] copy probe: 3 x (256 MiB read + 256 MiB written)", file=sys.stderr)
+        log("copy probe: 3 x (256 MiB read + 256 MiB written)")
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU SPIKE (one partition range per "
@@ -149,6 +287,11 @@ def main():
         return bench_batch(args, torch, hs)
     if args.config == "c5":
         return bench_serve(args, hs)
+    return bench_single(args, torch, hs)
+
+
+def bench_single(args, torch, hs):
+    """C1 / C2 / C3: one system, factor + solve per step."""
     cfg = CONFIGS[args.config]
     n, k, nrhs = cfg["n"], cfg["k"], cfg["nrhs"]
     t_gen = time.time()
@@ -181,109 +324,74 @@ def main():
             stats.append(s.stats())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        # resident factors (repeat-RHS solves)
-        tr = time.perf_counter()
+        tr = time.perf_counter()                       # resident factors (repeat-RHS solves)
         for _ in range(max(2, steps // 2)):
             s.solve_dev(d_b, d_x, nrhs)
         torch.cuda.synchronize()
         t_res = (time.perf_counter() - tr) / max(2, steps // 2)
-        st = stats[-1]
         x = d_x.cpu().numpy().T.copy(order="F")
-        return dict(wall=wall, lat=np.array(lat), stats=stats, last=st, x=x, resident_ms=t_res * 1e3, solver=s)
+        return dict(wall=wall, lat=np.array(lat), stats=stats, last=stats[-1], x=x, resident_ms=t_res * 1e3)
 
-    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)
     log("generated %s: n=%d nnz=%d in %.2f s" % (args.config, n, A.nnz, t_gen))
     mixed = run("mixed", 0, args.steps, args.warmup)
     log("mixed: %.3f ms/step, P=%d" % (mixed["wall"] / args.steps * 1e3, mixed["last"]["P"]))
     P_auto = mixed["last"]["P"]
     rhs_s = args.steps * nrhs / mixed["wall"]
 
-    out_fp64 = None
+    out_fp64, x64 = None, None
     if not args.no_fp64:
         f64 = run("fp64", 0, max(3, args.steps // 2), 1)
         out_fp64 = {"rhs_per_s": round(len(f64["lat"]) * nrhs / f64["wall"], 2),
                     "ms_per_step": round(f64["wall"] / len(f64["lat"]) * 1e3, 3), "partitions": f64["last"]["P"],
-                    "max_rel_residual": f64["last"]["max_rel_residual"],
-                    "refine_iters": f64["last"]["refine_iters"]}
+                    "max_rel_residual": f64["last"]["max_rel_residual"], "refine_iters": f64["last"]["refine_iters"]}
         if cfg["P_cfg"]:
             f64c = run("fp64", cfg["P_cfg"], 2, 1)
             out_fp64["configured_partitions"] = cfg["P_cfg"]
             out_fp64["rhs_per_s_configured_partitions"] = round(len(f64c["lat"]) * nrhs / f64c["wall"], 2)
             out_fp64["max_rel_residual_configured_partitions"] = f64c["last"]["max_rel_residual"]
+            mixc = run("mixed", cfg["P_cfg"], 2, 1)
+            out_fp64["mixed_rhs_per_s_configured_partitions"] = round(len(mixc["lat"]) * nrhs / mixc["wall"], 2)
         x64 = f64["x"]
         log("fp64: %s" % out_fp64)
-    else:
-        x64 = None
 
-    # ---- per-kernel event times averaged over the timed steps; dominant stage for the roofline
-    keys = {"factor": "ms_factor", "spikes": "ms_spikes", "recover": "ms_recover", "ysweep": "ms_ysweep",
-            "residual": "ms_residual"}
-    avg = {k: float(np.mean([st[v] for st in mixed["stats"]])) for k, v in keys.items()}
-    avg_other = {k: float(np.mean([st[k] for st in mixed["stats"]])) for k in
-                 ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_factor_total", "ms_solve_total")}
-    passes = max(1, mixed["last"]["refine_iters"] + 1)
-    ridge = PEAK_TFLOPS["f32"] * 1e12 / (HBM_PEAK_GBS * 1e9)
-    # every stage against both peaks; the binding one is the stage's arithmetic intensity side
-    # of the FP32 ridge point
-    stage_roof = {}
-    for st_name, st_ms in avg.items():
-        nl = passes if st_name in ("recover", "ysweep", "residual") else 1
-        if st_ms <= 0:
-            continue
-        f_st, b_st = algorithmic_counts(st_name, n, A.nnz, k, k, nrhs, P_auto, 4)
-        t_st = st_ms / nl * 1e-3
-        pk = PEAK_TFLOPS["f64"] if st_name == "residual" else PEAK_TFLOPS["f32"]
-        stage_roof[st_name] = {"launches": nl, "ms_per_launch": round(st_ms / nl, 4),
-                               "GB_s": round(b_st / t_st / 1e9, 2), "hbm_frac": round(b_st / t_st / 1e9 / HBM_PEAK_GBS, 4),
-                               "TFLOP_s": round(f_st / t_st / 1e12, 3), "mfma_frac": round(f_st / t_st / 1e12 / pk, 4),
-                               "bound": "mfma" if f_st / b_st >= pk * 1e12 / (HBM_PEAK_GBS * 1e9) else "hbm"}
-    dom = max(avg, key=avg.get)
-    launches = passes if dom in ("recover", "ysweep", "residual") else 1
-    F, Bytes = algorithmic_counts(dom, n, A.nnz, k, k, nrhs, P_auto, 4)
-    per_launch_ms = avg[dom] / launches
-    flop_rate = F / (per_launch_ms * 1e-3) / 1e12
-    byte_rate = Bytes / (per_launch_ms * 1e-3) / 1e9
-    inten = F / Bytes
-    if inten >= ridge:
-        roof = {"bound": "mfma", "achieved": round(flop_rate, 3), "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-                "frac": round(flop_rate / PEAK_TFLOPS["f32"], 4)}
-    else:
-        roof = {"bound": "hbm", "achieved": round(byte_rate, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(byte_rate / HBM_PEAK_GBS, 4)}
-    # instantiation the library launches for this shape (FP32 factors, b = 16 NT, column tiles per
-    # sweep workgroup limited to CT * NT <= 16)
-    nt = mixed["last"]["b"] // 16
-    ct_vw = min(4, 16 // nt)
-    ct_y = min(ct_vw, 1 if min(nrhs, 64) <= 16 else (2 if min(nrhs, 64) <= 32 else 4))
-    # residual: a lane holds a pair of RHS columns; team width covers ceil(ymax / 2) lanes
-    res_need = (min(nrhs, 64) + 1) // 2
-    res_tw = 4 if res_need <= 4 else (8 if res_need <= 8 else 16)
-    kname = {"factor": "k_factor<float,%d>" % nt, "spikes": "k_sweep<float,%d,%d,true>" % (nt, ct_vw),
-             "recover": "k_recover<float,%d,%d>" % (nt, ct_y), "ysweep": "k_sweep<float,%d,%d,false>" % (nt, ct_y),
-             "residual": "k_residual<double,%d,2>" % res_tw}[dom]
-    roof.update({"traffic": measured_traffic(args.config, kname), "kernel": kname,
-                 "stage": dom, "algorithmic_flops": F, "algorithmic_bytes": Bytes, "ms_per_launch": round(per_launch_ms, 4)})
+    avg = {st: float(np.mean([x[key] for x in mixed["stats"]])) for st, key in STAGES.items()}
+    avg_other = {kk: float(np.mean([x[kk] for x in mixed["stats"]])) for kk in
+                 ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_norms", "ms_factor_total",
+                  "ms_solve_total")}
+    passes = max(1, mixed["last"]["solve_passes"])
+    counts = {st: algorithmic_counts(st, n, A.nnz, k, k, nrhs, P_auto, 4) for st in STAGES}
+    names = kernel_names_for([mixed["last"]["b"] // 16], nrhs)
+    stage_roof = stage_roofline(avg, counts, passes, 1, args.config, names)
+    roof = roofline_of(stage_roof, avg)
 
-    # ---- CPU baseline: the oracle on the host cores, same matrix and RHS, configured partitions
     cpu = None
-    err64 = None
-    if not args.no_cpu:
+    if not args.no_cpu:          # the oracle at the configured partitions, on all cores and on one
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         oracle.build()
-        threads = min(16, os.cpu_count() or 1)
         Ao = oracle.CSR(A.n, A.rowptr, A.colind, A.aij)
         Pc = cfg["P_cfg"] or 32
+        info = cpu_info()
         t0 = time.perf_counter()
-        log("cpu oracle: %d threads, %d partitions ..." % (threads, Pc))
-        Xo, _ = oracle.spike_solve(Ao, B, Pc, nthreads=threads, want_reduced=False)
-        t_cpu = time.perf_counter() - t0
-        log("cpu oracle: %.2f s" % t_cpu)
-        cpu = {"value": round(nrhs / t_cpu, 3), "unit": "RHS/s", "cores": threads, "kind": "port",
-               "sample": "full workload: one oracle SPIKE solve (FP64, %d partitions, %d RHS) in %.2f s" % (Pc, nrhs, t_cpu)}
+        Xo, _ = oracle.spike_solve(Ao, B, Pc, nthreads=info["nproc"], want_reduced=False)
+        t_par = time.perf_counter() - t0
         x64 = Xo
-    if x64 is not None:
-        err64 = float(np.linalg.norm(mixed["x"] - x64) / np.linalg.norm(x64))
+        # one core: the same solve on a leading slice of the rows (partitions of the same size,
+        # about 10 s of work), scaled by the rows solved
+        per_row = t_par * min(info["nproc"], Pc) / n
+        ns = int(min(n, max(1000, 10.0 / max(per_row, 1e-12))))
+        Ps = max(1, ns // max(1, n // Pc))
+        Asl = oracle.extract_csr(Ao, 0, ns, 0, ns)
+        t0 = time.perf_counter()
+        oracle.spike_solve(Asl, np.asfortranarray(B[:ns]), Ps, nthreads=1, want_reduced=False)
+        t_ser = time.perf_counter() - t0
+        log("cpu oracle: %.2f s on %d cores; %d rows on one core in %.2f s" % (t_par, info["nproc"], ns, t_ser))
+        cpu = {"value": round(nrhs / t_par, 3), "unit": "RHS/s", "cores": info["nproc"], "kind": "port",
+               "sample": "full workload: one oracle SPIKE solve (FP64, %d partitions, %d RHS) on all %d host cores in "
+                         "%.2f s; one core: the first %d rows (%d partitions of the same size) in %.2f s, scaled by rows"
+                         % (Pc, nrhs, info["nproc"], t_par, ns, Ps, t_ser),
+               "serial_value": round(nrhs / (t_ser * n / ns), 3), "serial_cores": 1, **info}
+    err64 = float(np.linalg.norm(mixed["x"] - x64) / np.linalg.norm(x64)) if x64 is not None else None
 
     lat = mixed["lat"]
     line = {
@@ -299,10 +407,115 @@ def main():
         "fp64": out_fp64,
         "mixed_vs_fp64": round(rhs_s / out_fp64["rhs_per_s"], 3) if out_fp64 else None,
         "resident_factors_rhs_per_s": round(nrhs / (mixed["resident_ms"] * 1e-3), 2),
-        "stage_ms": {**{k: round(v, 4) for k, v in avg.items()}, **{k: round(v, 4) for k, v in avg_other.items()}},
+        "stage_ms": {**{kk: round(v, 4) for kk, v in avg.items()}, **{kk: round(v, 4) for kk, v in avg_other.items()}},
         "roofline": roof,
         "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+def bench_batch(args, torch, hs):
+    """C4: one batched factor + solve of many independent systems per step (spike_batch_solve_dev)."""
+    dev = torch.device("cuda:0")
+    spec = c4_systems(args.nsys, C4["seed"])
+    t_gen = time.time()
+    systems, meta = [], []
+    for (n, kk, nrhs, sa, sb) in spec:
+        A = hs.gen_banded(n, kk, kk, fill=C4["fill"], dom=(C4["dom"], C4["dom"]), diag_floor=1.0, seed=sa)
+        B = hs.gen_dense(n, nrhs, seed=sb)
+        systems.append(dict(n=n, nrhs=nrhs, rowptr=torch.from_numpy(A.rowptr).to(dev),
+                            colind=torch.from_numpy(A.colind).to(dev), aij=torch.from_numpy(A.aij).to(dev),
+                            b=torch.from_numpy(np.ascontiguousarray(B.T)).to(dev),
+                            x=torch.zeros((nrhs, n), dtype=torch.float64, device=dev)))
+        meta.append((n, A.nnz, kk, nrhs, sa, sb))
+        if len(meta) % 1024 == 0:
+            log("c4: %d of %d systems generated and uploaded" % (len(meta), len(spec)))
+    torch.cuda.synchronize()
+    rows = sum(m[0] for m in meta)
+    nnz = sum(m[1] for m in meta)
+    rhs = sum(m[3] for m in meta)
+    log("generated c4: %d systems, %d rows, nnz %d, %d RHS in %.1f s" % (len(systems), rows, nnz, rhs, time.time() - t_gen))
+
+    def run(precision, steps, warmup):
+        bt = hs.Batch(precision, tol=C4["tol"])
+        for _ in range(warmup):
+            bt.solve_dev(systems)
+        torch.cuda.synchronize()
+        lat, rels, sts = [], None, []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            rels, st = bt.solve_dev(systems)
+            lat.append((time.perf_counter() - ts) * 1e3)
+            sts.append(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bt.close()
+        return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts)
+
+    mixed = run("mixed", args.steps, args.warmup)
+    log("mixed: %.2f ms/step, max rel residual %.3e" % (mixed["wall"] / args.steps * 1e3, mixed["rel"].max()))
+    f64 = None
+    if not args.no_fp64:
+        r = run("fp64", max(2, args.steps // 2), 1)
+        f64 = {"rhs_per_s": round(len(r["lat"]) * rhs / r["wall"], 2), "ms_per_step": round(r["wall"] / len(r["lat"]) * 1e3, 3),
+               "max_rel_residual": float(r["rel"].max()),
+               "note": "the whole batch's FP64 factors exceed HBM: solved in windows of systems"}
+        log("fp64: %s" % f64)
+    rhs_s = args.steps * rhs / mixed["wall"]
+    avg = {st: float(np.mean([x[key] for x in mixed["stats"]])) for st, key in STAGES.items()}
+    avg_other = {kk: float(np.mean([x[kk] for x in mixed["stats"]])) for kk in
+                 ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_norms", "ms_factor_total",
+                  "ms_solve_total")}
+    last = mixed["stats"][-1]
+    passes, groups = max(1, last["solve_passes"]), max(1, last["launch_groups"])
+    counts = {st: [0.0, 0.0] for st in STAGES}
+    nts = set()
+    for (n, z, kk, nrhs, _, _) in meta:
+        P = hs.autotune_partitions(n, kk, kk, nrhs, precision=hs.SPIKE_PREC_MIXED)
+        nts.add(1 if kk <= 16 else (2 if kk <= 32 else 4))
+        for st in STAGES:
+            f, b = algorithmic_counts(st, n, z, kk, kk, nrhs, P, 4)
+            counts[st][0] += f
+            counts[st][1] += b
+    names = kernel_names_for(sorted(nts), C4["nrhs_hi"])
+    stage_roof = stage_roofline(avg, counts, passes, groups, "c4", names)
+    roof = roofline_of(stage_roof, avg)
+
+    cpu = None
+    if not args.no_cpu:          # the oracle on a bounded prefix of the batch, one system per core
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+
+        def job(i):
+            n, _, kk, nrhs, sa, sb = meta[i]
+
+            def prep():
+                A = hs.gen_banded(n, kk, kk, fill=C4["fill"], dom=(C4["dom"], C4["dom"]), diag_floor=1.0, seed=sa)
+                return oracle.CSR(A.n, A.rowptr, A.colind, A.aij), hs.gen_dense(n, nrhs, seed=sb)
+
+            def go(d, nthreads):
+                oracle.spike_solve(d[0], d[1], max(1, min(4, n // (4 * kk + 4))), nthreads=nthreads, want_reduced=False)
+            return prep, go
+        cpu = oracle_baseline([job(i) for i in range(len(meta))], [m[3] for m in meta], 20.0, 10.0,
+                              "oracle SPIKE FP64 (up to 4 partitions, the reference's P) per system, in batch order")
+    lat = mixed["lat"]
+    line = {
+        "metric": METRIC, "value": round(rhs_s, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(mixed["wall"] / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "C4: %d independent banded systems in one batched solve, n log-uniform [1e3, 2e5], "
+                               "half-bandwidth in {4,8,16,32,64}, 30%% fill, dominance 1.1, nrhs 1-8" % len(systems),
+                   "systems": len(systems), "rows": rows, "nnz": nnz, "nrhs_total": rhs, "tol": C4["tol"],
+                   "seed": C4["seed"], "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "accuracy": {"max_rel_residual": float(mixed["rel"].max()), "solve_passes": passes},
+        "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
+        "stage_ms": {**{kk: round(v, 4) for kk, v in avg.items()}, **{kk: round(v, 4) for kk, v in avg_other.items()}},
+        "roofline": roof, "stage_roofline": stage_roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     return 0
@@ -312,7 +525,6 @@ def bench_serve(args, hs):
     """C5: Poisson request stream through the request server (spike_server_*); throughput is the
     RHS solved over the span from the first arrival to the last completion."""
     import threading
-    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)
     reqs, mats = c5_stream(args.nreq, C5["seed"])
     t_gen = time.time()
     A = []
@@ -365,37 +577,31 @@ def bench_serve(args, hs):
                     hits=sum(x["resident_hit"] for x in infos))
 
     mixed = run("mixed")
-    log("mixed: span %.2f s, p50 %.1f ms, p99 %.1f ms, failed %d" % (mixed["span"], np.percentile(mixed["lat"], 50),
-                                                                      np.percentile(mixed["lat"], 99), mixed["failed"]))
+    log("mixed: span %.2f s, p50 %.1f ms, p99 %.1f ms, failed %d, oom retries %d, estimate coverage %.3f" %
+        (mixed["span"], np.percentile(mixed["lat"], 50), np.percentile(mixed["lat"], 99), mixed["failed"],
+         mixed["stats"]["oom_retries"], mixed["stats"]["est_ratio_max"]))
     f64 = None
     if not args.no_fp64:
         r = run("fp64")
         f64 = {"rhs_per_s": round(rhs / r["span"], 2), "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
                "p99_ms": round(float(np.percentile(r["lat"], 99)), 3), "failed": r["failed"],
-               "max_rel_residual": r["maxrel"]}
+               "max_rel_residual": r["maxrel"], "oom_retries": r["stats"]["oom_retries"]}
         log("fp64: %s" % f64)
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         oracle.build()
-        threads = min(16, os.cpu_count() or 1)
-        t0, done, cnt = time.perf_counter(), 0, 0
-        for i, (_, mid, nrhs, _) in enumerate(reqs):
-            a = A[mid]
-            k = mats[mid][2]
-            if a.nnz > 20_000_000:          # keep the sample bounded: requests up to 2e7 nonzeros
-                continue
-            oracle.spike_solve(oracle.CSR(a.n, a.rowptr, a.colind, a.aij), Bs[i], max(1, min(4, a.n // (4 * k + 4))),
-                               nthreads=threads, want_reduced=False)
-            done += nrhs
-            cnt += 1
-            if time.perf_counter() - t0 > 15.0:
-                break
-        t_cpu = time.perf_counter() - t0
-        cpu = {"value": round(done / t_cpu, 3), "unit": "RHS/s", "cores": threads, "kind": "port",
-               "sample": "%d of %d requests in arrival order, those up to 2e7 nonzeros (%d RHS), solved one after "
-                         "another by the oracle (FP64 SPIKE, up to 4 partitions), %.1f s" % (cnt, len(reqs), done, t_cpu)}
+        order = [i for i, (_, mid, _, _) in enumerate(reqs) if A[mid].nnz <= 20_000_000]
+
+        def job(i):
+            _, mid, nrhs, _ = reqs[i]
+            a, k = A[mid], mats[mid][2]
+            return (lambda: oracle.CSR(a.n, a.rowptr, a.colind, a.aij),
+                    lambda d, nthreads: oracle.spike_solve(d, Bs[i], max(1, min(4, a.n // (4 * k + 4))),
+                                                           nthreads=nthreads, want_reduced=False))
+        cpu = oracle_baseline([job(i) for i in order], [reqs[i][2] for i in order], 20.0, 10.0,
+                              "requests in arrival order up to 2e7 nonzeros, oracle SPIKE FP64 (up to 4 partitions)")
     rhs_s = rhs / mixed["span"]
     line = {
         "metric": METRIC, "value": round(rhs_s, 2), "unit": "RHS/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
@@ -410,114 +616,10 @@ def bench_serve(args, hs):
                        "p99": round(float(np.percentile(mixed["lat"], 99)), 3)},
         "accuracy": {"max_rel_residual": mixed["maxrel"], "failed": mixed["failed"]},
         "serving": {"resident_hits": mixed["hits"], "peak_device_bytes": mixed["stats"]["peak_bytes"],
-                    "budget_bytes": mixed["stats"]["budget_bytes"], "span_s": round(mixed["span"], 3)},
+                    "budget_bytes": mixed["stats"]["budget_bytes"], "span_s": round(mixed["span"], 3),
+                    "oom_retries": mixed["stats"]["oom_retries"], "estimate_coverage_max": mixed["stats"]["est_ratio_max"]},
         "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
         "roofline": None, "cpu_baseline": cpu,
-    }
-    print(json.dumps(line))
-    return 0
-
-
-def bench_batch(args, torch, hs):
-    """C4: one batched factor + solve of many independent systems per step (spike_batch_solve_dev)."""
-    log = lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True)
-    dev = torch.device("cuda:0")
-    spec = c4_systems(args.nsys, C4["seed"])
-    t_gen = time.time()
-    systems, host = [], []
-    for (n, k, nrhs, sa, sb) in spec:
-        A = hs.gen_banded(n, k, k, fill=C4["fill"], dom=(C4["dom"], C4["dom"]), diag_floor=1.0, seed=sa)
-        B = hs.gen_dense(n, nrhs, seed=sb)
-        systems.append(dict(n=n, nrhs=nrhs, rowptr=torch.from_numpy(A.rowptr).to(dev),
-                            colind=torch.from_numpy(A.colind).to(dev), aij=torch.from_numpy(A.aij).to(dev),
-                            b=torch.from_numpy(np.ascontiguousarray(B.T)).to(dev),
-                            x=torch.zeros((nrhs, n), dtype=torch.float64, device=dev)))
-        host.append((A, B, k))
-        if len(host) % 512 == 0:
-            log("c4: %d of %d systems generated and uploaded" % (len(host), len(spec)))
-    torch.cuda.synchronize()
-    rows = sum(s["n"] for s in systems)
-    nnz = sum(A.nnz for A, _, _ in host)
-    rhs = sum(s["nrhs"] for s in systems)
-    log("generated c4: %d systems, %d rows, nnz %d, %d RHS in %.1f s" % (len(systems), rows, nnz, rhs, time.time() - t_gen))
-
-    def run(precision, steps, warmup):
-        bt = hs.Batch(precision, tol=C4["tol"])
-        for _ in range(warmup):
-            bt.solve_dev(systems)
-        torch.cuda.synchronize()
-        lat, rels, sts = [], None, []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ts = time.perf_counter()
-            rels, st = bt.solve_dev(systems)
-            lat.append((time.perf_counter() - ts) * 1e3)
-            sts.append(st)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        bt.close()
-        return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts)
-
-    mixed = run("mixed", args.steps, args.warmup)
-    log("mixed: %.2f ms/step, max rel residual %.3e" % (mixed["wall"] / args.steps * 1e3, mixed["rel"].max()))
-    f64 = None
-    if not args.no_fp64:
-        r = run("fp64", max(2, args.steps // 2), 1)
-        f64 = {"rhs_per_s": round(len(r["lat"]) * rhs / r["wall"], 2), "ms_per_step": round(r["wall"] / len(r["lat"]) * 1e3, 3),
-               "max_rel_residual": float(r["rel"].max())}
-        log("fp64: %s" % f64)
-    rhs_s = args.steps * rhs / mixed["wall"]
-    keys = {"factor": "ms_factor", "spikes": "ms_spikes", "recover": "ms_recover", "ysweep": "ms_ysweep",
-            "residual": "ms_residual"}
-    avg = {k: float(np.mean([st[v] for st in mixed["stats"]])) for k, v in keys.items()}
-    passes = max(1, max(st["refine_iters"] for st in mixed["stats"]) + 1)
-    dom = max(avg, key=avg.get)
-    launches = passes if dom in ("recover", "ysweep", "residual") else 1
-    F = Bts = 0.0
-    for (A, B, k) in host:
-        P = hs.autotune_partitions(A.n, k, k, B.shape[1], precision=hs.SPIKE_PREC_MIXED)
-        f, b = algorithmic_counts(dom, A.n, A.nnz, k, k, B.shape[1], P, 4)
-        F += f
-        Bts += b
-    per_launch_ms = avg[dom] / launches
-    byte_rate = Bts / (per_launch_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(byte_rate, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(byte_rate / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": {"factor": "k_factor", "spikes": "k_sweep (V/W)", "recover": "k_recover", "ysweep": "k_sweep (RHS)",
-                       "residual": "k_residual"}[dom] + " (all block sizes of the batch)",
-            "stage": dom, "algorithmic_flops": F, "algorithmic_bytes": Bts, "ms_per_launch": round(per_launch_ms, 4)}
-    cpu = None
-    if not args.no_cpu:      # the oracle on a bounded sample of the batch, scaled to RHS/s
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        oracle.build()
-        threads = min(16, os.cpu_count() or 1)
-        t0, done, cnt = time.perf_counter(), 0, 0
-        for (A, B, k) in host:
-            oracle.spike_solve(oracle.CSR(A.n, A.rowptr, A.colind, A.aij), B, max(1, min(4, A.n // (4 * k + 4))),
-                               nthreads=threads, want_reduced=False)
-            done += B.shape[1]
-            cnt += 1
-            if time.perf_counter() - t0 > 15.0:
-                break
-        t_cpu = time.perf_counter() - t0
-        cpu = {"value": round(done / t_cpu, 3), "unit": "RHS/s", "cores": threads, "kind": "port",
-               "sample": "first %d of %d systems (%d RHS), oracle SPIKE FP64 with up to 4 partitions, %.1f s" %
-                         (cnt, len(host), done, t_cpu)}
-    lat = mixed["lat"]
-    line = {
-        "metric": METRIC, "value": round(rhs_s, 2), "unit": "RHS/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(mixed["wall"] / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": "C4: %d independent banded systems, n log-uniform [1e3, 2e5], half-bandwidth in "
-                               "{4,8,16,32,64}, 30%% fill, dominance 1.1, nrhs 1-8" % len(systems),
-                   "systems": len(systems), "rows": rows, "nnz": nnz, "nrhs_total": rhs, "tol": C4["tol"],
-                   "seed": C4["seed"], "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
-        "accuracy": {"max_rel_residual": float(mixed["rel"].max())},
-        "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
-        "stage_ms": {k: round(v, 4) for k, v in avg.items()},
-        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
     return 0

@@ -50,7 +50,8 @@ class Stats(ctypes.Structure):
         ("ms_red_setup", ctypes.c_double), ("ms_ysweep", ctypes.c_double), ("ms_reduced", ctypes.c_double),
         ("ms_recover", ctypes.c_double), ("ms_residual", ctypes.c_double),
         ("ms_factor_total", ctypes.c_double), ("ms_solve_total", ctypes.c_double),
-        ("device_bytes", ctypes.c_size_t), ("reduced_direct", ctypes.c_int),
+        ("device_bytes", ctypes.c_size_t), ("reduced_direct", ctypes.c_int), ("ms_norms", ctypes.c_double),
+        ("solve_passes", ctypes.c_int), ("launch_groups", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -89,6 +90,7 @@ class ServerStats(ctypes.Structure):
         ("rhs_solved", ctypes.c_int64), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
         ("mean_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double), ("resident_matrices", ctypes.c_int64),
         ("resident_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+        ("oom_retries", ctypes.c_int64), ("est_ratio_max", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -939,7 +939,8 @@ int Engine::solve_chunk(int c0, int ymax) {
     const size_t hs = h_bytes(dtype);
     const double rtol = (o.precision == SPIKE_PREC_FP64) ? o.reduced_tol : std::max(o.reduced_tol, 1e-9);
     const int max_refine = std::max(0, o.max_refine);
-    double t_y = 0, t_red = 0, t_rec = 0, t_res = 0;
+    double t_y = 0, t_red = 0, t_rec = 0, t_res = 0, t_norm = 0;
+    int passes = 0;
     // ---- per-group workspaces, descriptors and items
     std::vector<std::vector<int>> active(ngroups);
     for (int gi = 0; gi < ngroups; ++gi) {
@@ -1156,12 +1157,16 @@ int Engine::solve_chunk(int c0, int ymax) {
             HIPCK(hipGetLastError());
         }
         HIPCK(hipEventRecord(ev[11], s));
-        // ---- FP64 residual of every column
+        // ---- FP64 residual of every column (residual launches timed apart from the norm sums)
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
             if ((e = launch_residual(dtype, ymax, ymax, true, (unsigned)g.chunks.size(), s, g.d_chunks.as<RowChunk>(),
                                      g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.d_active.as<int>())))
                 return e;
+        }
+        HIPCK(hipEventRecord(ev[13], s));
+        for (int gi = 0; gi < ngroups; ++gi) {
+            Group &g = *groups[gi];
             hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(g.sys.size() * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
                                g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.norms.as<double>(), ymax);
             HIPCK(hipGetLastError());
@@ -1191,12 +1196,16 @@ int Engine::solve_chunk(int c0, int ymax) {
             if (changed && any)
                 if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
         }
-        t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 12);
+        t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 13); t_norm += ev_ms(13, 12);
+        ++passes;
     }
     st.ms_ysweep = t_y;
     st.ms_reduced = t_red;
     st.ms_recover = t_rec;
     st.ms_residual = t_res;
+    st.ms_norms = t_norm;
+    st.solve_passes = passes;
+    st.launch_groups = ngroups;
     return SPIKE_SUCCESS;
 }
 
@@ -1227,7 +1236,7 @@ void stage_times(spike_stats_t &dst, const spike_stats_t &src, bool add) {
     double spike_stats_t::*f[] = {&spike_stats_t::ms_convert, &spike_stats_t::ms_factor, &spike_stats_t::ms_rtop,
                                   &spike_stats_t::ms_spikes, &spike_stats_t::ms_red_setup, &spike_stats_t::ms_ysweep,
                                   &spike_stats_t::ms_reduced, &spike_stats_t::ms_recover, &spike_stats_t::ms_residual,
-                                  &spike_stats_t::ms_factor_total, &spike_stats_t::ms_solve_total};
+                                  &spike_stats_t::ms_factor_total, &spike_stats_t::ms_solve_total, &spike_stats_t::ms_norms};
     for (auto m : f) dst.*m = add ? dst.*m + src.*m : src.*m;
 }
 

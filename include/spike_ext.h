@@ -76,11 +76,14 @@ typedef struct {
     double  ms_ysweep;         /* RHS tips, summed over refinement passes (k_sweep)          */
     double  ms_reduced;        /* reduced-system iterations, summed                          */
     double  ms_recover;        /* recovery sweeps, summed (k_recover)                        */
-    double  ms_residual;       /* FP64 residuals, summed (k_residual)                        */
+    double  ms_residual;       /* FP64 residual launches, summed (k_residual)                */
     double  ms_factor_total;   /* spike_factor wall time                                     */
     double  ms_solve_total;    /* spike_solve_dev wall time                                  */
     size_t  device_bytes;      /* HBM held by the handle                                     */
     int     reduced_direct;    /* systems whose reduced system the direct band LU solved      */
+    double  ms_norms;          /* per-column residual norm sums, summed (k_norm_reduce)      */
+    int     solve_passes;      /* solve + refinement passes of the last column chunk         */
+    int     launch_groups;     /* block-size groups: launches of each stage kernel per pass  */
 } spike_stats_t;
 
 typedef struct spike_handle spike_handle_t;
@@ -229,6 +232,9 @@ typedef struct {
     double   p50_ms, p99_ms, mean_ms, elapsed_ms;
     int64_t  resident_matrices;
     uint64_t resident_bytes, peak_bytes, budget_bytes;
+    int64_t  oom_retries;         /* jobs that ran out of device memory and were run again        */
+    double   est_ratio_max;       /* largest (device bytes a fresh job held) / (its admission
+                                     estimate); <= 1 means every estimate covered its job       */
 } spike_server_stats_t;
 
 typedef struct spike_server spike_server_t;

@@ -121,12 +121,12 @@ class Solver:
     """Factor-once / solve-many GPU SPIKE handle (include/spike_ext.h)."""
 
     def __init__(self, precision="fp64", partitions=0, tol=1e-10, max_refine=8, reduced_maxit=64,
-                 reduced_tol=1e-15, verbose=False, reduced_solver="auto"):
+                 reduced_tol=1e-15, verbose=False, reduced_solver="auto", graph=False):
         self._L = load()
         self._opts = _lib.default_opts(precision=_prec_code(precision), partitions=int(partitions), tol=float(tol),
                                        max_refine=int(max_refine), reduced_maxit=int(reduced_maxit),
                                        reduced_tol=float(reduced_tol), verbose=int(bool(verbose)),
-                                       reduced_solver=_red_code(reduced_solver))
+                                       reduced_solver=_red_code(reduced_solver), graph=int(bool(graph)))
         h = ctypes.c_void_p()
         check(self._L.spike_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_create")
         self._h = h
@@ -225,10 +225,10 @@ class Solver:
 class Batch:
     """Factor and solve many independent device-resident systems together (include/spike_ext.h)."""
 
-    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, verbose=False, partitions=0):
+    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, verbose=False, partitions=0, graph=False):
         self._L = load()
         self._opts = _lib.default_opts(precision=_prec_code(precision), tol=float(tol), max_refine=int(max_refine),
-                                       verbose=int(bool(verbose)), partitions=int(partitions))
+                                       verbose=int(bool(verbose)), partitions=int(partitions), graph=int(bool(graph)))
         h = ctypes.c_void_p()
         check(self._L.spike_batch_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_batch_create")
         self._h = h

@@ -36,6 +36,7 @@ class Opts(ctypes.Structure):
         ("precision", ctypes.c_int), ("partitions", ctypes.c_int), ("max_refine", ctypes.c_int),
         ("tol", ctypes.c_double), ("reduced_maxit", ctypes.c_int), ("reduced_tol", ctypes.c_double),
         ("sweep_waves", ctypes.c_int), ("verbose", ctypes.c_int), ("reduced_solver", ctypes.c_int),
+        ("graph", ctypes.c_int),
     ]
 
 
@@ -51,7 +52,7 @@ class Stats(ctypes.Structure):
         ("ms_recover", ctypes.c_double), ("ms_residual", ctypes.c_double),
         ("ms_factor_total", ctypes.c_double), ("ms_solve_total", ctypes.c_double),
         ("device_bytes", ctypes.c_size_t), ("reduced_direct", ctypes.c_int), ("ms_norms", ctypes.c_double),
-        ("solve_passes", ctypes.c_int), ("launch_groups", ctypes.c_int),
+        ("solve_passes", ctypes.c_int), ("launch_groups", ctypes.c_int), ("graph_replayed", ctypes.c_int),
     ]
 
     def as_dict(self):

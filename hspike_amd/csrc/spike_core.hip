@@ -290,6 +290,10 @@ struct Group {
     std::vector<BandSys> rd_bs;
     std::vector<RedDirect> rd_desc_h;
     DevBuf rd_desc, rd_pass, rd_bsd, rd_ab, rd_piv, rd_x, rd_info, rd_items;
+    // device-side solve control
+    DevBuf d_sysstat, d_summary, d_direct;
+    std::vector<int> direct_h;              // 1: the system's reduced system is solved directly
+    bool direct_dirty = true;
 };
 
 struct Engine {
@@ -379,6 +383,8 @@ struct Engine {
     ~Engine() {
         if (events)
             for (auto &e : ev) (void)hipEventDestroy(e);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
     }
     int init_events() {
         if (events) return 0;
@@ -402,9 +408,36 @@ struct Engine {
         }
         return s;
     }
+    // Graph mode captures work on a non-blocking stream of the engine's own when the caller gave
+    // none (the legacy default stream cannot be captured); the caller's default-stream work is
+    // waited for first, as the default stream itself would have ordered it.
+    int prepare_stream() {
+        if (o.graph && !stream) {
+            if (!own_stream) HIPCK(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+            stream = own_stream;
+        }
+        if (own_stream && stream == own_stream) HIPCK(hipStreamSynchronize(nullptr));
+        return 0;
+    }
     int factor();
     int reduced_direct(Group &g, const std::vector<int> &which, DevBuf &xo, int ymax);
+    int direct_pass(Group &g, int ymax);
+    int launch_jacobi(Group &g, int ncc, int ymax, const DevBuf &xo, DevBuf &xn);
+    int run_passes(int ymax, bool capture, double rtol, int max_refine, double &t_y, double &t_red, double &t_rec,
+                   double &t_res, double &t_norm, int &passes);
+    int read_stats(bool *incomplete = nullptr);
     int solve_chunk(int c0, int m);
+    int solve_setup(int c0, int ymax, bool &cached);
+    int solve_graph(int ymax);
+    std::vector<uint64_t> graph_key(int ymax) const;
+    // graph of one single-chunk solve (replayed while graph_key() is unchanged); the budget of
+    // passes and sweep pairs comes from the last eager solve
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> gkey;
+    int graph_passes = 0, graph_pairs = 0;
+    int eager_pairs = 0;                     // largest sweep-pair count of a pass in the last eager solve
+    hipStream_t own_stream = nullptr;        // non-blocking stream for graph mode when the caller gave none
+    bool graph_replayed = false;
     int solve(int nrhs_hint);
 };
 
@@ -486,13 +519,13 @@ namespace {
 int Engine::factor() {
     if (sys.empty()) { set_err("no matrix bound%s", ""); return SPIKE_ERR_INVALID_ARG; }
     int e = init_events();
-    if (e) return e;
+    if (e || (e = prepare_stream())) return e;
     const double t0 = now_ms();
     hipStream_t s = stream;
     const int nsys = (int)sys.size();
     // a new factorization invalidates the direct solves of the reduced systems
     for (auto &S : sys) S.red_direct = false;
-    for (auto &gp : groups) { gp->rd_factored = false; gp->rd_sys.clear(); }
+    for (auto &gp : groups) { gp->rd_factored = false; gp->rd_sys.clear(); gp->direct_h.clear(); gp->direct_dirty = true; }
     HIPCK(hipEventRecord(ev[0], s));
 
     // ---- validation and bandwidth of every system (one launch)
@@ -933,16 +966,13 @@ int Engine::reduced_direct(Group &g, const std::vector<int> &which, DevBuf &xo, 
 }
 
 // ============================================================================ solve
-int Engine::solve_chunk(int c0, int ymax) {
+// Workspaces, per-system views and sweep items of an RHS chunk (uploaded only when they change).
+int Engine::solve_setup(int c0, int ymax, bool &cached_all) {
     hipStream_t s = stream;
     int e;
     const size_t hs = h_bytes(dtype);
-    const double rtol = (o.precision == SPIKE_PREC_FP64) ? o.reduced_tol : std::max(o.reduced_tol, 1e-9);
-    const int max_refine = std::max(0, o.max_refine);
-    double t_y = 0, t_red = 0, t_rec = 0, t_res = 0, t_norm = 0;
-    int passes = 0;
+    cached_all = true;
     // ---- per-group workspaces, descriptors and items
-    std::vector<std::vector<int>> active(ngroups);
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         const size_t np = g.parts.size(), nif = g.ifs.size(), ngs = g.sys.size();
@@ -960,7 +990,6 @@ int Engine::solve_chunk(int c0, int ymax) {
         for (size_t gs = 0; gs < ngs; ++gs) rrows += sys[g.sys[gs]].n;
         const int ldrm = (ymax + 1) & ~1;      // even: a lane's column pair is one aligned load
         if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(rrows * ldrm * hs))) return e;
-        active[gi].assign(ngs, 0);
         std::vector<uint64_t> key = {(uint64_t)c0, (uint64_t)ymax, (uint64_t)(uintptr_t)g.rwork.p,
                                      (uint64_t)(uintptr_t)g.xrm.p};
         for (size_t gs = 0; gs < ngs; ++gs) {
@@ -982,14 +1011,26 @@ int Engine::solve_chunk(int c0, int ymax) {
             sd.xrm = (char *)g.xrm.p + roff * ldrm * hs;
             sd.ldrm = ldrm;
             roff += S.n;
-            active[gi][gs] = m > 0;
             S.refine_iters = 0;
-            S.maxrel_before = c0 == 0 ? 0.0 : S.maxrel;
         }
-        if (cached) {
-            if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
-            continue;
+        if ((e = g.d_active.ensure(ngs * sizeof(int))) || (e = g.d_red_active.ensure(ngs * sizeof(int))) ||
+            (e = g.d_sysstat.ensure(ngs * sizeof(SysStat))) || (e = g.d_summary.ensure(4 * sizeof(int))))
+            return e;
+        if (g.direct_h.size() != ngs) {              // systems on the direct reduced solve (set anew after a factor)
+            g.direct_h.assign(ngs, 0);
+            if (o.reduced_solver == SPIKE_RED_DIRECT)
+                for (size_t gs = 0; gs < ngs; ++gs) {
+                    SysHost &S = sys[g.sys[gs]];
+                    if (S.P > 1 && S.ku + S.kl > 0) { g.direct_h[gs] = 1; S.red_direct = true; }
+                }
+            g.direct_dirty = true;
         }
+        if (g.direct_dirty) {
+            if ((e = put_small(g.d_direct, g.direct_h.data(), ngs * sizeof(int)))) return e;
+            g.direct_dirty = false;
+        }
+        if (cached) continue;
+        cached_all = false;
         if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
         // items: Y sweeps and recoveries, from b (first pass) and from r (refinement passes)
         g.ct_y = ct_for(ymax, ct_max(g.tcode, g.NT));
@@ -1020,15 +1061,161 @@ int Engine::solve_chunk(int c0, int ymax) {
             (e = upload(g.items_rb, rb_.data(), rb_.size() * sizeof(SweepItem), s)) ||
             (e = upload(g.items_rr, rr_.data(), rr_.size() * sizeof(SweepItem), s)))
             return e;
-        if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
-        if ((e = g.d_red_active.ensure(ngs * sizeof(int)))) return e;
         g.solve_key = key;
     }
+    return SPIKE_SUCCESS;
+}
 
-    bool any = true;
-    for (int it = 0; any && it <= max_refine; ++it) {
-        double ta = now_ms();
-        HIPCK(hipEventRecord(ev[8], s));
+int Engine::solve_chunk(int c0, int ymax) {
+    hipStream_t s = stream;
+    int e;
+    const double rtol = (o.precision == SPIKE_PREC_FP64) ? o.reduced_tol : std::max(o.reduced_tol, 1e-9);
+    const int max_refine = std::max(0, o.max_refine);
+    double t_y = 0, t_red = 0, t_rec = 0, t_res = 0, t_norm = 0;
+    int passes = 0;
+    bool cached;
+    if ((e = solve_setup(c0, ymax, cached))) return e;
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const int ngs = (int)g.sys.size();
+        hipLaunchKernelGGL(k_stat_init, dim3((ngs + 255) / 256), dim3(256), 0, s, g.d_sysstat.as<SysStat>(), ngs,
+                           (const SysDev *)g.d_sysdev.as<SysDev>(), g.d_active.as<int>(), c0 == 0 ? 1 : 0);
+        HIPCK(hipGetLastError());
+    }
+    e = run_passes(ymax, false, rtol, max_refine, t_y, t_red, t_rec, t_res, t_norm, passes);
+    if (e) return e;
+    st.ms_ysweep = t_y;
+    st.ms_reduced = t_red;
+    st.ms_recover = t_rec;
+    st.ms_residual = t_res;
+    st.ms_norms = t_norm;
+    st.solve_passes = passes;
+    st.launch_groups = ngroups;
+    return read_stats();
+}
+
+// Per-system results of the last solve chunk into the host view.  *incomplete (optional) is set
+// when a system would have needed more than the launches made: still active after the last pass,
+// or a reduced iteration that ran out of sweeps or diverged.
+int Engine::read_stats(bool *incomplete) {
+    if (incomplete) *incomplete = false;
+    for (int gi = 0; gi < ngroups; ++gi) {
+        Group &g = *groups[gi];
+        const size_t ngs = g.sys.size();
+        std::vector<SysStat> h(ngs);
+        std::vector<int> act(ngs);
+        int e;
+        if ((e = get_small({{h.data(), g.d_sysstat.p, ngs * sizeof(SysStat)}, {act.data(), g.d_active.p, ngs * sizeof(int)}})))
+            return e;
+        for (size_t gs = 0; gs < ngs; ++gs) {
+            SysHost &S = sys[g.sys[gs]];
+            S.maxrel = h[gs].maxrel;
+            S.refine_iters = h[gs].refine_iters;
+            S.red_iters = h[gs].red_iters;
+            S.red_update = h[gs].red_update;
+            if (incomplete && (act[gs] || h[gs].unfinished || h[gs].red_active || h[gs].diverged)) *incomplete = true;
+        }
+    }
+    return SPIKE_SUCCESS;
+}
+
+std::vector<uint64_t> Engine::graph_key(int ymax) const {
+    auto bits = [](double v) { uint64_t u; memcpy(&u, &v, sizeof u); return u; };
+    std::vector<uint64_t> k = {(uint64_t)ymax, (uint64_t)graph_passes, (uint64_t)graph_pairs, (uint64_t)dtype,
+                               (uint64_t)o.max_refine, bits(o.tol), bits(o.reduced_tol), (uint64_t)o.precision,
+                               (uint64_t)o.reduced_maxit, (uint64_t)(uintptr_t)stream, (uint64_t)ngroups};
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const Group &g = *groups[gi];
+        k.insert(k.end(), {(uint64_t)g.NT, (uint64_t)g.tcode, (uint64_t)g.ct_y, (uint64_t)g.n_items_y, (uint64_t)g.n_items_r,
+                           (uint64_t)g.ifs.size(), (uint64_t)g.chunks.size(), (uint64_t)g.sys.size(), (uint64_t)g.ymax});
+        const DevBuf *bufs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
+                                &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.partials, &g.norms, &g.conv, &g.rwork,
+                                &g.xrm, &g.d_active, &g.d_red_active, &g.d_sysstat, &g.d_direct, &g.d_summary,
+                                &g.items_yb, &g.items_yr, &g.items_rb, &g.items_rr, &g.d_parts, &g.d_ifs, &g.d_part_iface,
+                                &g.d_chunks, &g.d_chunk0, &g.d_sysdev};
+        for (auto b : bufs) k.push_back((uint64_t)(uintptr_t)b->p);
+        k.insert(k.end(), g.solve_key.begin(), g.solve_key.end());
+    }
+    return k;
+}
+
+// One single-chunk solve as a graph replay: the launches of the eager solve (stat init, then
+// graph_passes passes of graph_pairs sweep pairs) captured once and replayed while graph_key()
+// holds.  SPIKE_ERR_UNSUPPORTED sends the caller to the eager solve: no budget yet, descriptors
+// just changed, a system on the direct reduced solve, or a replay that needed more launches than
+// the budget (then the graph is dropped and the eager solve sets a new budget).
+int Engine::solve_graph(int ymax) {
+    if (graph_passes <= 0 || graph_pairs <= 0) return SPIKE_ERR_UNSUPPORTED;
+    for (int gi = 0; gi < ngroups; ++gi)
+        if (std::any_of(groups[gi]->direct_h.begin(), groups[gi]->direct_h.end(), [](int d) { return d != 0; }))
+            return SPIKE_ERR_UNSUPPORTED;
+    bool cached;
+    int e;
+    if ((e = solve_setup(0, ymax, cached))) return e;
+    if (!cached) return SPIKE_ERR_UNSUPPORTED;
+    const double rtol = (o.precision == SPIKE_PREC_FP64) ? o.reduced_tol : std::max(o.reduced_tol, 1e-9);
+    const int max_refine = std::max(0, o.max_refine);
+    const std::vector<uint64_t> key = graph_key(ymax);
+    if (!gexec || key != gkey) {
+        if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        hipGraph_t graph = nullptr;
+        HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = SPIKE_SUCCESS;
+        for (int gi = 0; gi < ngroups && !rc; ++gi) {
+            Group &g = *groups[gi];
+            const int ngs = (int)g.sys.size();
+            hipLaunchKernelGGL(k_stat_init, dim3((ngs + 255) / 256), dim3(256), 0, stream, g.d_sysstat.as<SysStat>(), ngs,
+                               (const SysDev *)g.d_sysdev.p, g.d_active.as<int>(), 1);
+            if (hipGetLastError() != hipSuccess) rc = SPIKE_ERR_DEVICE;
+        }
+        double t_y = 0, t_red = 0, t_rec = 0, t_res = 0, t_norm = 0;
+        int passes = 0;
+        if (!rc) rc = run_passes(ymax, true, rtol, max_refine, t_y, t_red, t_rec, t_res, t_norm, passes);
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        if (rc || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            if (!rc) set_err("graph capture failed (%s)", hipGetErrorString(ce));
+            return rc ? rc : SPIKE_ERR_DEVICE;
+        }
+        const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            (void)hipGetLastError();
+            gexec = nullptr;
+            set_err("graph instantiation failed (%s)", hipGetErrorString(ie));
+            return SPIKE_ERR_DEVICE;
+        }
+        gkey = key;
+    }
+    HIPCK(hipGraphLaunch(gexec, stream));
+    bool incomplete = false;
+    if ((e = read_stats(&incomplete))) return e;
+    if (incomplete) {
+        (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        return SPIKE_ERR_UNSUPPORTED;
+    }
+    st.ms_ysweep = st.ms_reduced = st.ms_recover = st.ms_residual = st.ms_norms = 0.0;   // not timed per stage
+    st.solve_passes = graph_passes;
+    st.launch_groups = ngroups;
+    graph_replayed = true;
+    return SPIKE_SUCCESS;
+}
+
+// The solve and refinement passes of one RHS chunk.  Every decision is taken on the device
+// (k_red_check, k_refine_check); kernels skip finished systems.  Eager (capture = false): the
+// host reads a 16-byte summary after each pair of sweeps and each pass to stop early and to move
+// diverging reduced systems to the direct solve.  Capture: a fixed budget of graph_passes passes
+// and graph_pairs pairs of sweeps, no host call at all.
+int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, double &t_y, double &t_red, double &t_rec,
+                       double &t_res, double &t_norm, int &passes) {
+    hipStream_t s = stream;
+    int e;
+    const int npass = capture ? graph_passes : max_refine + 1;
+    auto rec = [&](int i) -> int { if (!capture) HIPCK(hipEventRecord(ev[i], s)); return 0; };
+    for (int it = 0; it < npass; ++it) {
+        if ((e = rec(8))) return e;
         // ---- y tips
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -1043,106 +1230,76 @@ int Engine::solve_chunk(int c0, int ymax) {
             })));
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[9], s));
-        // ---- reduced system: block-Jacobi sweeps in FP64 until each system's update is small
-        std::vector<DevBuf *> redcur(ngroups, nullptr);
+        if ((e = rec(9))) return e;
+        // ---- reduced system: block-Jacobi sweeps in FP64, two per convergence check; the last
+        // iterate of every system is in red0 (an even number of sweeps)
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
-            if (g.ifs.empty()) { redcur[gi] = &g.red0; continue; }
-            const size_t nif = g.ifs.size(), ngs = g.sys.size();
+            if (g.ifs.empty()) continue;
+            const int nif = (int)g.ifs.size(), ngs = (int)g.sys.size();
             const int ncc = ceil_div(ymax, 16);
-            const size_t slot_elems = (size_t)2 * g.b * ymax;
-            if (dtype == SPIKE_DTYPE_Z) { if ((e = fill_zero<zd>(g.red0, nif * slot_elems, s))) return e; }
-            else if ((e = fill_zero<double>(g.red0, nif * slot_elems, s))) return e;
-            // systems of this pass on the direct band LU: forced by the options, or switched
-            // there because their block-Jacobi sweeps did not contract earlier in this factorization
-            std::vector<int> ract = active[gi], direct;
-            for (size_t gs = 0; gs < ngs; ++gs) {
-                if (!ract[gs]) continue;
-                SysHost &S = sys[g.sys[gs]];
-                if (S.ku + S.kl == 0) continue;
-                if (o.reduced_solver == SPIKE_RED_DIRECT || S.red_direct) {
-                    direct.push_back((int)gs);
-                    ract[gs] = 0;
-                    S.red_direct = true;
-                    S.red_iters = 0;
-                    S.red_update = 0.0;
+            const int slot = 2 * g.b * ymax;
+            const dim3 zg((unsigned)std::min(nif, 65536));
+            if (dtype == SPIKE_DTYPE_Z)
+                hipLaunchKernelGGL(k_red_zero<zd>, zg, dim3(256), 0, s, (const IfaceDesc *)g.d_ifs.p, nif,
+                                   (const int *)g.d_active.p, g.red0.as<zd>(), slot);
+            else
+                hipLaunchKernelGGL(k_red_zero<double>, zg, dim3(256), 0, s, (const IfaceDesc *)g.d_ifs.p, nif,
+                                   (const int *)g.d_active.p, g.red0.as<double>(), slot);
+            hipLaunchKernelGGL(k_red_begin, dim3((ngs + 255) / 256), dim3(256), 0, s, g.d_sysstat.as<SysStat>(), ngs,
+                               (const SysDev *)g.d_sysdev.p, (const int *)g.d_active.p, (const int *)g.d_direct.p,
+                               g.d_red_active.as<int>());
+            HIPCK(hipGetLastError());
+            const int maxpairs = capture ? graph_pairs : std::max(1, o.reduced_maxit / 2);
+            bool sweeping = std::find(g.direct_h.begin(), g.direct_h.end(), 0) != g.direct_h.end();
+            int summary[4] = {0, 0, 0, 0};
+            int pairs_done = 0;
+            for (int pr = 0; sweeping && pr < maxpairs; ++pr, ++pairs_done) {
+                for (int rep = 0; rep < 2; ++rep) {
+                    const DevBuf &xo = rep == 0 ? g.red0 : g.red1;
+                    DevBuf &xn = rep == 0 ? g.red1 : g.red0;
+                    if ((e = launch_jacobi(g, ncc, ymax, xo, xn))) return e;
                 }
-            }
-            const bool auto_direct = o.reduced_solver == SPIKE_RED_AUTO;
-            if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
-            DevBuf *xo = &g.red0, *xn = &g.red1;
-            std::vector<double> conv(nif * ncc * 2);
-            std::vector<double> dmax(ngs), vmax(ngs);
-            int iters = 0;
-            const int maxit = std::max(2, o.reduced_maxit);
-            const bool any_jacobi = std::any_of(ract.begin(), ract.end(), [](int a) { return a != 0; });
-            while (any_jacobi && iters < maxit) {
-                // two sweeps per convergence check (the first sweep from zero is the truncated
-                // solution; the second measures the coupling); systems stop once converged
-                for (int rep = 0; rep < 2 && iters < maxit; ++rep) {
-                    if (dtype == SPIKE_DTYPE_Z) {
-                        NT_SWITCH(g.NT, {
-                            hipLaunchKernelGGL((k_red_jacobi<zd, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
-                                               g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p, (const zd *)g.vb.p,
-                                               (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p,
-                                               (const zd *)g.yb.p, (const zd *)g.T.p, (const zd *)xo->p, xn->as<zd>(),
-                                               g.conv.as<double>(), ymax, ymax, g.d_red_active.as<int>());
-                        });
-                    } else {
-                        NT_SWITCH(g.NT, {
-                            if constexpr (NT <= 4)
-                                hipLaunchKernelGGL((k_red_jacobi_q<double, NT>), dim3((unsigned)(nif * ncc)), dim3(NT * 64), 0, s,
-                                                   g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
-                                                   (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
-                                                   (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
-                                                   (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
-                                                   ymax, g.d_red_active.as<int>());
-                            else
-                                hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3((unsigned)(nif * ncc)), dim3(256), 0, s,
-                                                   g.d_ifs.as<IfaceDesc>(), ncc, (const double *)g.vt.p,
-                                                   (const double *)g.vb.p, (const double *)g.wt.p, (const double *)g.wb.p,
-                                                   (const double *)g.yt.p, (const double *)g.yb.p, (const double *)g.T.p,
-                                                   (const double *)xo->p, xn->as<double>(), g.conv.as<double>(), ymax,
-                                                   ymax, g.d_red_active.as<int>());
-                        });
-                    }
+                hipLaunchKernelGGL(k_red_check, dim3((unsigned)ngs), dim3(64), 0, s, g.d_sysstat.as<SysStat>(), ngs,
+                                   (const SysDev *)g.d_sysdev.p, (const int *)g.d_part_iface.p, (const double *)g.conv.p,
+                                   ncc, rtol, 2 * (pr + 1), g.d_red_active.as<int>());
+                HIPCK(hipGetLastError());
+                if (!capture) {
+                    hipLaunchKernelGGL(k_stat_summary, dim3(1), dim3(256), 0, s, (const SysStat *)g.d_sysstat.p, ngs,
+                                       (const int *)g.d_active.p, g.d_summary.as<int>());
                     HIPCK(hipGetLastError());
-                    std::swap(xo, xn);
-                    ++iters;
+                    if ((e = get_small({{summary, g.d_summary.p, sizeof summary}}))) return e;
+                    sweeping = summary[0] > 0;
                 }
-                if ((e = get_small({{conv.data(), g.conv.p, conv.size() * sizeof(double)}}))) return e;
-                std::fill(dmax.begin(), dmax.end(), 0.0);
-                std::fill(vmax.begin(), vmax.end(), 0.0);
-                for (size_t q = 0; q < nif * ncc; ++q) {
-                    const int gs = g.ifs[q / ncc].sys;
-                    dmax[gs] = std::max(dmax[gs], conv[2 * q]);
-                    vmax[gs] = std::max(vmax[gs], conv[2 * q + 1]);
-                }
-                for (size_t gs = 0; gs < ngs; ++gs) {
-                    if (!ract[gs]) continue;
-                    SysHost &S = sys[g.sys[gs]];
-                    const double rel = vmax[gs] > 0 ? dmax[gs] / vmax[gs] : dmax[gs];
-                    S.red_iters = iters;
-                    S.red_update = rel;
-                    if (rel <= rtol) ract[gs] = 0;                  // converged
-                    else if (!std::isfinite(rel) || rel > 1e3) {    // diverging
-                        ract[gs] = 0;
-                        if (auto_direct) { direct.push_back((int)gs); S.red_direct = true; }
-                    }
-                }
-                if (!std::any_of(ract.begin(), ract.end(), [](int a) { return a != 0; })) break;
-                if (iters >= maxit) break;
-                if ((e = put_small(g.d_red_active, ract.data(), ngs * sizeof(int)))) return e;
             }
-            if (auto_direct)                                         // not converged within maxit sweeps
-                for (size_t gs = 0; gs < ngs; ++gs)
-                    if (ract[gs]) { direct.push_back((int)gs); sys[g.sys[gs]].red_direct = true; }
-            if (!direct.empty() && (e = reduced_direct(g, direct, *xo, ymax))) return e;
-            redcur[gi] = xo;     // last written iterate
-            g.red_last = (xo == &g.red1) ? 1 : 0;
+            g.red_last = 0;
+            if (!capture) eager_pairs = std::max(eager_pairs, pairs_done);
+            if (capture || o.reduced_solver != SPIKE_RED_AUTO) {
+                if (!capture && std::any_of(g.direct_h.begin(), g.direct_h.end(), [](int d) { return d != 0; }))
+                    if ((e = direct_pass(g, ymax))) return e;
+                continue;
+            }
+            // diverged, or still sweeping after reduced_maxit sweeps: the direct solve from now on
+            if (summary[0] > 0 || summary[1] > 0) {
+                std::vector<SysStat> h(ngs);
+                std::vector<int> act(ngs);
+                if ((e = get_small({{h.data(), g.d_sysstat.p, ngs * sizeof(SysStat)}, {act.data(), g.d_active.p, ngs * sizeof(int)}})))
+                    return e;
+                for (int gs = 0; gs < ngs; ++gs)
+                    if (act[gs] && (h[gs].diverged || h[gs].red_active)) {
+                        g.direct_h[gs] = 1;
+                        sys[g.sys[gs]].red_direct = true;
+                        g.direct_dirty = true;
+                    }
+                if (g.direct_dirty) {
+                    if ((e = put_small(g.d_direct, g.direct_h.data(), ngs * sizeof(int)))) return e;
+                    g.direct_dirty = false;
+                }
+            }
+            if (std::any_of(g.direct_h.begin(), g.direct_h.end(), [](int d) { return d != 0; }))
+                if ((e = direct_pass(g, ymax))) return e;
         }
-        HIPCK(hipEventRecord(ev[10], s));
+        if ((e = rec(10))) return e;
         // ---- recovery into the FP64 solution
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -1151,12 +1308,12 @@ int Engine::solve_chunk(int c0, int ymax) {
             T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
                 (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.U.as<T>(),
-                                          g.Wf.as<T>(), g.Vf.as<T>(), (const H *)redcur[gi]->p, g.d_part_iface.as<int>(),
+                                          g.Wf.as<T>(), g.Vf.as<T>(), (const H *)g.red0.p, g.d_part_iface.as<int>(),
                                           g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0, g.d_active.as<int>());
             })));
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[11], s));
+        if ((e = rec(11))) return e;
         // ---- FP64 residual of every column (residual launches timed apart from the norm sums)
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
@@ -1164,48 +1321,76 @@ int Engine::solve_chunk(int c0, int ymax) {
                                      g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.d_active.as<int>())))
                 return e;
         }
-        HIPCK(hipEventRecord(ev[13], s));
+        if ((e = rec(13))) return e;
         for (int gi = 0; gi < ngroups; ++gi) {
             Group &g = *groups[gi];
-            hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(g.sys.size() * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
+            const int ngs = (int)g.sys.size();
+            hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)(ngs * ymax)), dim3(256), 0, s, g.d_chunk0.as<int>(),
                                g.d_sysdev.as<SysDev>(), g.partials.as<double>(), g.norms.as<double>(), ymax);
+            hipLaunchKernelGGL(k_refine_check, dim3((ngs + 255) / 256), dim3(256), 0, s, g.d_sysstat.as<SysStat>(), ngs,
+                               (const SysDev *)g.d_sysdev.p, (const double *)g.norms.p, ymax, o.tol, it, max_refine,
+                               g.d_active.as<int>());
             HIPCK(hipGetLastError());
         }
-        HIPCK(hipEventRecord(ev[12], s));
-        any = false;
-        for (int gi = 0; gi < ngroups; ++gi) {
-            Group &g = *groups[gi];
-            const size_t ngs = g.sys.size();
-            std::vector<double> nr(ngs * ymax * 2);
-            if ((e = get_small({{nr.data(), g.norms.p, nr.size() * sizeof(double)}}))) return e;
-            bool changed = false;
-            for (size_t gs = 0; gs < ngs; ++gs) {
-                if (!active[gi][gs]) continue;
-                SysHost &S = sys[g.sys[gs]];
-                double mr = 0.0;
-                for (int c = 0; c < g.sysdev[gs].m; ++c) {
-                    const double rr = nr[(gs * ymax + c) * 2], bn = nr[(gs * ymax + c) * 2 + 1];
-                    const double rel = bn > 0 ? sqrt(rr / bn) : sqrt(rr);
-                    mr = std::isfinite(rel) ? std::max(mr, rel) : INFINITY;
-                }
-                S.refine_iters = it;
-                S.maxrel = std::max(S.maxrel_before, mr);     // this chunk's latest pass only
-                if (mr <= o.tol || it == max_refine || !std::isfinite(mr)) { active[gi][gs] = 0; changed = true; }
-                else any = true;
-            }
-            if (changed && any)
-                if ((e = put_small(g.d_active, active[gi].data(), ngs * sizeof(int)))) return e;
-        }
-        t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 13); t_norm += ev_ms(13, 12);
+        if ((e = rec(12))) return e;
         ++passes;
+        if (!capture) {
+            t_y += ev_ms(8, 9); t_red += ev_ms(9, 10); t_rec += ev_ms(10, 11); t_res += ev_ms(11, 13); t_norm += ev_ms(13, 12);
+            bool any = false;
+            for (int gi = 0; gi < ngroups && !any; ++gi) {
+                Group &g = *groups[gi];
+                int summary[4];
+                hipLaunchKernelGGL(k_stat_summary, dim3(1), dim3(256), 0, s, (const SysStat *)g.d_sysstat.p,
+                                   (int)g.sys.size(), (const int *)g.d_active.p, g.d_summary.as<int>());
+                HIPCK(hipGetLastError());
+                if ((e = get_small({{summary, g.d_summary.p, sizeof summary}}))) return e;
+                any = summary[2] > 0;
+            }
+            if (!any) break;
+        }
     }
-    st.ms_ysweep = t_y;
-    st.ms_reduced = t_red;
-    st.ms_recover = t_rec;
-    st.ms_residual = t_res;
-    st.ms_norms = t_norm;
-    st.solve_passes = passes;
-    st.launch_groups = ngroups;
+    return SPIKE_SUCCESS;
+}
+
+// One pass of the direct reduced solve for the group's active systems on it.
+int Engine::direct_pass(Group &g, int ymax) {
+    std::vector<int> act(g.sys.size());
+    int e;
+    if ((e = get_small({{act.data(), g.d_active.p, act.size() * sizeof(int)}}))) return e;
+    std::vector<int> which;
+    for (size_t gs = 0; gs < g.sys.size(); ++gs)
+        if (act[gs] && g.direct_h[gs]) which.push_back((int)gs);
+    if (which.empty()) return SPIKE_SUCCESS;
+    return reduced_direct(g, which, g.red0, ymax);
+}
+
+int Engine::launch_jacobi(Group &g, int ncc, int ymax, const DevBuf &xo, DevBuf &xn) {
+    hipStream_t s = stream;
+    const unsigned nb = (unsigned)(g.ifs.size() * ncc);
+    if (dtype == SPIKE_DTYPE_Z) {
+        NT_SWITCH(g.NT, {
+            hipLaunchKernelGGL((k_red_jacobi<zd, NT>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p,
+                               (const zd *)g.vb.p, (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p, (const zd *)g.yb.p,
+                               (const zd *)g.T.p, (const zd *)xo.p, xn.as<zd>(), g.conv.as<double>(), ymax, ymax,
+                               g.d_red_active.as<int>());
+        });
+    } else {
+        NT_SWITCH(g.NT, {
+            if constexpr (NT <= 4)
+                hipLaunchKernelGGL((k_red_jacobi_q<double, NT>), dim3(nb), dim3(NT * 64), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
+                                   (const double *)g.vt.p, (const double *)g.vb.p, (const double *)g.wt.p,
+                                   (const double *)g.wb.p, (const double *)g.yt.p, (const double *)g.yb.p,
+                                   (const double *)g.T.p, (const double *)xo.p, xn.as<double>(), g.conv.as<double>(), ymax,
+                                   ymax, g.d_red_active.as<int>());
+            else
+                hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
+                                   (const double *)g.vt.p, (const double *)g.vb.p, (const double *)g.wt.p,
+                                   (const double *)g.wb.p, (const double *)g.yt.p, (const double *)g.yb.p,
+                                   (const double *)g.T.p, (const double *)xo.p, xn.as<double>(), g.conv.as<double>(), ymax,
+                                   ymax, g.d_red_active.as<int>());
+        });
+    }
+    HIPCK(hipGetLastError());
     return SPIKE_SUCCESS;
 }
 
@@ -1213,14 +1398,33 @@ int Engine::solve(int) {
     if (!factored) {
         int e = factor();
         if (e) return e;
+    } else if (int e = prepare_stream()) {
+        return e;
     }
     const double t0 = now_ms();
     int maxn = 0;
     for (auto &S : sys) { maxn = std::max(maxn, S.nrhs); S.maxrel = 0.0; }
     const int chunk = 64;
-    for (int c0 = 0; c0 < maxn; c0 += chunk) {
-        const int e = solve_chunk(c0, std::min(chunk, maxn - c0));
-        if (e) return e;
+    graph_replayed = false;
+    bool done = false;
+    if (o.graph && maxn > 0 && maxn <= chunk) {
+        const int e = solve_graph(maxn);
+        if (e == SPIKE_SUCCESS) done = true;
+        else if (e != SPIKE_ERR_UNSUPPORTED) return e;
+    }
+    if (!done) {
+        eager_pairs = 0;
+        for (int c0 = 0; c0 < maxn; c0 += chunk) {
+            const int e = solve_chunk(c0, std::min(chunk, maxn - c0));
+            if (e) return e;
+        }
+        // budget of the graph of the next single-chunk solve: this solve's passes, one spare sweep pair
+        bool direct = false;
+        for (auto &S : sys) direct = direct || S.red_direct;
+        if (o.graph && maxn <= chunk && !direct) {
+            graph_passes = st.solve_passes;
+            graph_pairs = eager_pairs + 1;
+        }
     }
     st.ms_solve_total = now_ms() - t0;
     for (auto &S : sys)
@@ -1255,6 +1459,7 @@ void fill_stats(const Engine &E, spike_stats_t *s) {
     s->device_bytes = E.device_bytes();
     s->reduced_direct = 0;
     for (auto &X : E.sys) s->reduced_direct += X.red_direct ? 1 : 0;
+    s->graph_replayed = E.graph_replayed ? 1 : 0;
 }
 
 }  // namespace
@@ -1556,6 +1761,7 @@ int spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_sy
     bt->windows = 0;
     const double t0 = now_ms();
     int e = SPIKE_SUCCESS;
+    const int graph = E.o.graph;      // windows rebind every system: replay only a batch solved whole
     for (int lo = 0; lo < nsys;) {
         const int hi = (int)std::min<int64_t>(nsys, (int64_t)lo + bt->cap);
         E.dtype = dtype;
@@ -1567,8 +1773,10 @@ int spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_sy
             S.nrhs = q.nrhs; S.d_b = q.d_b; S.ldb = q.n; S.d_x = q.d_x; S.ldx = q.n;
         }
         E.factored = false;
+        E.o.graph = (lo == 0 && hi == nsys) ? graph : 0;
         e = E.factor();
         if (!e) e = E.solve(0);
+        E.o.graph = graph;
         if (e == SPIKE_ERR_OOM && hi - lo > 1) {      // free this window's arrays, retry it halved
             (void)hipStreamSynchronize(E.stream);
             E.groups.clear();

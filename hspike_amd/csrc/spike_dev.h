@@ -252,3 +252,18 @@ struct RowChunk {
     int sys;
     int row0;
 };
+
+// Per-system state of a solve kept on the device: the decisions of the refinement passes and of
+// the reduced iteration are made by small kernels from it, so a solve's launches need no host
+// round trip (eager runs read a summary only to stop early) and can be captured in a graph.
+struct SysStat {
+    double maxrel;          // largest column residual after the latest pass (with earlier chunks)
+    double maxrel_before;   // of the earlier RHS chunks
+    double red_update;      // last relative update of the reduced iteration
+    int red_iters;          // sweeps of the latest reduced iteration
+    int refine_iters;       // refinement passes applied
+    int red_active;         // still sweeping in this pass
+    int diverged;           // the sweeps stopped contracting in this pass
+    int unfinished;         // an earlier pass ended with the sweeps still running or diverged
+    int pad_;
+};

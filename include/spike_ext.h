@@ -46,7 +46,19 @@ typedef struct {
     int    sweep_waves;    /* waves (16 columns each) per sweep workgroup: 0 auto, 1, 2, 4 */
     int    verbose;        /* 1 = stage timings on stderr                                   */
     int    reduced_solver; /* SPIKE_RED_AUTO (default), SPIKE_RED_JACOBI or SPIKE_RED_DIRECT  */
+    int    graph;          /* 1: replay the solve phase from a captured HIP graph (see below)  */
 } spike_opts_t;
+
+/* Graph replay (opts.graph = 1).  The decisions of a solve -- which systems need another
+ * refinement pass, which reduced iterations have converged -- are taken on the device, so a
+ * solve is a fixed sequence of launches.  With graph = 1, a solve of at most 64 RHS columns is
+ * captured once (after an eager solve of the same shapes has set the budget of passes and sweeps)
+ * and replayed while the handle's or batch's arrays, shapes and options stay the same; the
+ * results are the eager solve's bits.  A replay that would need more passes or sweeps than its
+ * budget, a system on the direct reduced solve, or any change of the arrays falls back to the
+ * eager solve (which sets a new budget).  The handle then works on a non-blocking stream of its
+ * own unless spike_set_stream gave one; each call first waits for the default stream.  Per-stage
+ * times (ms_ysweep .. ms_norms) are not measured in a replayed solve. */
 
 /* Solver of the reduced system R (stage 3).  AUTO runs block-Jacobi sweeps and switches a
  * system to the direct solver when its sweeps stop contracting (update growing past 1e3 times
@@ -84,6 +96,7 @@ typedef struct {
     double  ms_norms;          /* per-column residual norm sums, summed (k_norm_reduce)      */
     int     solve_passes;      /* solve + refinement passes of the last column chunk         */
     int     launch_groups;     /* block-size groups: launches of each stage kernel per pass  */
+    int     graph_replayed;    /* 1: the last solve was a replay of a captured graph          */
 } spike_stats_t;
 
 typedef struct spike_handle spike_handle_t;

@@ -218,3 +218,44 @@ def test_backend_plugin_abi(hs, oracle_mod):
     X = np.zeros_like(B, order="F")
     assert L.directSolver_Solve(A.n, A.nnz, 2, p(A.colind), p(A.rowptr), p(A.aij), p(X), p(B)) == 0
     assert hs.residual(A, X, B).max() <= 1e-10
+
+
+def cross_panel_matrix(hs, n, k, shift, seed=0):
+    """Zero diagonal, 2 at (i, i xor shift), band noise with row sums <= 0.8: nonsingular (singular
+    values >= 1.2), but with shift = 4 every 4x4 pivot block of the block inversion holds only the
+    noise (its pivots lie in the neighbouring panel), and with shift = 16 every diagonal 16 x 16
+    block does (its pivots lie in the neighbouring block)."""
+    rng = np.random.default_rng(seed)
+    i = np.repeat(np.arange(n), 2 * k + 1)
+    j = i + np.tile(np.arange(-k, k + 1), n)
+    keep = (j >= 0) & (j < n)
+    i, j = i[keep], j[keep]
+    v = 0.8 / (2 * k) * rng.uniform(-1, 1, i.size)
+    v[j == i] = 0.0
+    v[j == (i ^ shift)] = 2.0
+    rp = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rp, i + 1, 1)
+    return hs.CSRMatrix(n, np.cumsum(rp).astype(np.int32), j.astype(np.int32), v)
+
+
+@pytest.mark.parametrize("shift,k,prec", [(4, 8, "fp64"), (4, 8, "mixed"), (4, 20, "fp64"), (16, 24, "fp64"),
+                                          (16, 24, "mixed")])
+def test_pivots_outside_the_pivot_panels(hs, oracle_mod, shift, k, prec):
+    """The block LU interchanges rows only inside each 4x4 pivot block (no pivoting across panels or
+    blocks, unlike the reference's SuperLU with threshold 1.0, spike_superlu.c:76).  A nonsingular
+    matrix whose pivots lie outside their panel must either come back right (the FP64 residual and
+    refinement vouch for it) or fail cleanly with SPIKE_ERR_SINGULAR / SPIKE_ERR_NOT_CONVERGED --
+    never rc = 0 with a wrong x."""
+    A = cross_panel_matrix(hs, 4000, k, shift, seed=shift + k)
+    B = hs.gen_dense(A.n, 2, seed=3)
+    s = hs.Solver(prec, partitions=4, max_refine=4)
+    s.set_matrix(A)
+    try:
+        s.factor()
+        X = s.solve(B)
+    except hs.SpikeError as ex:
+        assert ex.code in (-4, -5), ex
+        return
+    assert hs.residual(A, X, B).max() <= 1e-10
+    Xo = oracle_mod.direct_solve(oracle_mod.CSR(A.n, A.rowptr, A.colind, A.aij), B)
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 1e-9

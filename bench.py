@@ -23,8 +23,10 @@ the same workload: on every core at once (independent solves spread over the cor
 on one core -- the reference's driver loops over its partitions serially,
 OpenMP/spike_interfaces.c:105-237), and one solve after another on one core.
 
-Every stage is reported against both peaks (`stage_roofline`); `roofline` is the stage with the
-largest time per step.  Per-launch times are the library's HIP events on the stream it launches
+The timed steps run the factorization and then the solve replayed from a captured HIP graph (the
+library's mode for repeated solves of one shape, bit-identical to its eager solve); the eager
+throughput is reported beside it.  Every stage is reported against both peaks (`stage_roofline`,
+from eager steps of the same workload); `roofline` is the stage with the largest time per step.  Per-launch times are the library's HIP events on the stream it launches
 on; `traffic` (HBM bytes per launch) and MFMA busy cycles come from the committed rocprofv3
 counter summaries in profiles/ for this workload.
 
@@ -307,8 +309,8 @@ def bench_single(args, torch, hs):
     d_x = torch.zeros_like(d_b)
     torch.cuda.synchronize()
 
-    def run(precision, partitions, steps, warmup):
-        s = hs.Solver(precision, partitions=partitions, tol=cfg["tol"], max_refine=8)
+    def run(precision, partitions, steps, warmup, graph=False):
+        s = hs.Solver(precision, partitions=partitions, tol=cfg["tol"], max_refine=8, graph=graph)
         s.set_matrix_dev(n, d_rowptr, d_colind, d_aij)
         lat, stats = [], []
         for _ in range(warmup):
@@ -333,14 +335,19 @@ def bench_single(args, torch, hs):
         return dict(wall=wall, lat=np.array(lat), stats=stats, last=stats[-1], x=x, resident_ms=t_res * 1e3)
 
     log("generated %s: n=%d nnz=%d in %.2f s" % (args.config, n, A.nnz, t_gen))
-    mixed = run("mixed", 0, args.steps, args.warmup)
-    log("mixed: %.3f ms/step, P=%d" % (mixed["wall"] / args.steps * 1e3, mixed["last"]["P"]))
+    # timed steps: factor + the solve replayed from its captured graph; per-stage times from
+    # eager steps of the same solve (a replay does not time its stages)
+    mixed = run("mixed", 0, args.steps, max(2, args.warmup), graph=True)
+    eager = run("mixed", 0, max(2, args.steps // 2), 1)
+    log("mixed: %.3f ms/step replayed (%d of %d steps), %.3f ms/step eager, P=%d" %
+        (mixed["wall"] / args.steps * 1e3, sum(x["graph_replayed"] for x in mixed["stats"]), args.steps,
+         eager["wall"] / len(eager["lat"]) * 1e3, mixed["last"]["P"]))
     P_auto = mixed["last"]["P"]
     rhs_s = args.steps * nrhs / mixed["wall"]
 
     out_fp64, x64 = None, None
     if not args.no_fp64:
-        f64 = run("fp64", 0, max(3, args.steps // 2), 1)
+        f64 = run("fp64", 0, max(3, args.steps // 2), 2, graph=True)
         out_fp64 = {"rhs_per_s": round(len(f64["lat"]) * nrhs / f64["wall"], 2),
                     "ms_per_step": round(f64["wall"] / len(f64["lat"]) * 1e3, 3), "partitions": f64["last"]["P"],
                     "max_rel_residual": f64["last"]["max_rel_residual"], "refine_iters": f64["last"]["refine_iters"]}
@@ -354,11 +361,11 @@ def bench_single(args, torch, hs):
         x64 = f64["x"]
         log("fp64: %s" % out_fp64)
 
-    avg = {st: float(np.mean([x[key] for x in mixed["stats"]])) for st, key in STAGES.items()}
-    avg_other = {kk: float(np.mean([x[kk] for x in mixed["stats"]])) for kk in
+    avg = {st: float(np.mean([x[key] for x in eager["stats"]])) for st, key in STAGES.items()}
+    avg_other = {kk: float(np.mean([x[kk] for x in eager["stats"]])) for kk in
                  ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_norms", "ms_factor_total",
                   "ms_solve_total")}
-    passes = max(1, mixed["last"]["solve_passes"])
+    passes = max(1, eager["last"]["solve_passes"])
     counts = {st: algorithmic_counts(st, n, A.nnz, k, k, nrhs, P_auto, 4) for st in STAGES}
     names = kernel_names_for([mixed["last"]["b"] // 16], nrhs)
     stage_roof = stage_roofline(avg, counts, passes, 1, args.config, names)
@@ -407,6 +414,9 @@ def bench_single(args, torch, hs):
         "fp64": out_fp64,
         "mixed_vs_fp64": round(rhs_s / out_fp64["rhs_per_s"], 3) if out_fp64 else None,
         "resident_factors_rhs_per_s": round(nrhs / (mixed["resident_ms"] * 1e-3), 2),
+        "graph": {"replayed_steps": int(sum(x["graph_replayed"] for x in mixed["stats"])),
+                  "eager_rhs_per_s": round(len(eager["lat"]) * nrhs / eager["wall"], 2),
+                  "eager_ms_per_step": round(eager["wall"] / len(eager["lat"]) * 1e3, 3)},
         "stage_ms": {**{kk: round(v, 4) for kk, v in avg.items()}, **{kk: round(v, 4) for kk, v in avg_other.items()}},
         "roofline": roof,
         "stage_roofline": stage_roof,
@@ -438,8 +448,8 @@ def bench_batch(args, torch, hs):
     rhs = sum(m[3] for m in meta)
     log("generated c4: %d systems, %d rows, nnz %d, %d RHS in %.1f s" % (len(systems), rows, nnz, rhs, time.time() - t_gen))
 
-    def run(precision, steps, warmup):
-        bt = hs.Batch(precision, tol=C4["tol"])
+    def run(precision, steps, warmup, graph=False):
+        bt = hs.Batch(precision, tol=C4["tol"], graph=graph, tune="throughput")
         for _ in range(warmup):
             bt.solve_dev(systems)
         torch.cuda.synchronize()
@@ -455,26 +465,31 @@ def bench_batch(args, torch, hs):
         bt.close()
         return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts)
 
-    mixed = run("mixed", args.steps, args.warmup)
-    log("mixed: %.2f ms/step, max rel residual %.3e" % (mixed["wall"] / args.steps * 1e3, mixed["rel"].max()))
+    # timed steps: factor + the solve replayed from its captured graph; per-stage times from
+    # eager steps (a replay does not time its stages)
+    mixed = run("mixed", args.steps, max(2, args.warmup), graph=True)
+    eager = run("mixed", max(2, args.steps // 2), 1)
+    log("mixed: %.2f ms/step replayed (%d of %d steps), %.2f ms/step eager, max rel residual %.3e" %
+        (mixed["wall"] / args.steps * 1e3, sum(x["graph_replayed"] for x in mixed["stats"]), args.steps,
+         eager["wall"] / len(eager["lat"]) * 1e3, mixed["rel"].max()))
     f64 = None
     if not args.no_fp64:
-        r = run("fp64", max(2, args.steps // 2), 1)
+        r = run("fp64", max(2, args.steps // 2), 1, graph=True)
         f64 = {"rhs_per_s": round(len(r["lat"]) * rhs / r["wall"], 2), "ms_per_step": round(r["wall"] / len(r["lat"]) * 1e3, 3),
                "max_rel_residual": float(r["rel"].max()),
                "note": "the whole batch's FP64 factors exceed HBM: solved in windows of systems"}
         log("fp64: %s" % f64)
     rhs_s = args.steps * rhs / mixed["wall"]
-    avg = {st: float(np.mean([x[key] for x in mixed["stats"]])) for st, key in STAGES.items()}
-    avg_other = {kk: float(np.mean([x[kk] for x in mixed["stats"]])) for kk in
+    avg = {st: float(np.mean([x[key] for x in eager["stats"]])) for st, key in STAGES.items()}
+    avg_other = {kk: float(np.mean([x[kk] for x in eager["stats"]])) for kk in
                  ("ms_convert", "ms_rtop", "ms_red_setup", "ms_reduced", "ms_norms", "ms_factor_total",
                   "ms_solve_total")}
-    last = mixed["stats"][-1]
+    last = eager["stats"][-1]
     passes, groups = max(1, last["solve_passes"]), max(1, last["launch_groups"])
     counts = {st: [0.0, 0.0] for st in STAGES}
     nts = set()
     for (n, z, kk, nrhs, _, _) in meta:
-        P = hs.autotune_partitions(n, kk, kk, nrhs, precision=hs.SPIKE_PREC_MIXED)
+        P = hs.autotune_partitions(n, kk, kk, nrhs, precision=hs.SPIKE_PREC_MIXED, tune="throughput")
         nts.add(1 if kk <= 16 else (2 if kk <= 32 else 4))
         for st in STAGES:
             f, b = algorithmic_counts(st, n, z, kk, kk, nrhs, P, 4)
@@ -510,10 +525,14 @@ def bench_batch(args, torch, hs):
         "config": {"workload": "C4: %d independent banded systems in one batched solve, n log-uniform [1e3, 2e5], "
                                "half-bandwidth in {4,8,16,32,64}, 30%% fill, dominance 1.1, nrhs 1-8" % len(systems),
                    "systems": len(systems), "rows": rows, "nnz": nnz, "nrhs_total": rhs, "tol": C4["tol"],
-                   "seed": C4["seed"], "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},
+                   "seed": C4["seed"], "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update",
+                   "partitions": "autotuned for throughput (about 96 blocks per partition)"},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "accuracy": {"max_rel_residual": float(mixed["rel"].max()), "solve_passes": passes},
         "fp64": f64, "mixed_vs_fp64": round(rhs_s / f64["rhs_per_s"], 3) if f64 else None,
+        "graph": {"replayed_steps": int(sum(x["graph_replayed"] for x in mixed["stats"])),
+                  "eager_rhs_per_s": round(len(eager["lat"]) * rhs / eager["wall"], 2),
+                  "eager_ms_per_step": round(eager["wall"] / len(eager["lat"]) * 1e3, 3)},
         "stage_ms": {**{kk: round(v, 4) for kk, v in avg.items()}, **{kk: round(v, 4) for kk, v in avg_other.items()}},
         "roofline": roof, "stage_roofline": stage_roof, "cpu_baseline": cpu,
     }

@@ -93,8 +93,21 @@ def gen_dense(n, m, lo=-1.0, hi=1.0, seed=0, complex=False):
     return out
 
 
-def autotune_partitions(n, ku, kl, nrhs=1, complex=False, precision=SPIKE_PREC_FP64):
-    return load().spike_autotune_partitions(n, ku, kl, nrhs, SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D, precision)
+SPIKE_TUNE_LATENCY, SPIKE_TUNE_THROUGHPUT = 0, 1
+
+
+def _tune_code(tune):
+    codes = {"latency": SPIKE_TUNE_LATENCY, "throughput": SPIKE_TUNE_THROUGHPUT}
+    if tune in codes.values():
+        return int(tune)
+    if tune not in codes:
+        raise ValueError("tune must be 'latency' or 'throughput'")
+    return codes[tune]
+
+
+def autotune_partitions(n, ku, kl, nrhs=1, complex=False, precision=SPIKE_PREC_FP64, tune="latency"):
+    return load().spike_autotune_partitions_tuned(n, ku, kl, nrhs, SPIKE_DTYPE_Z if complex else SPIKE_DTYPE_D,
+                                                  precision, _tune_code(tune))
 
 
 def _prec_code(precision):
@@ -121,12 +134,13 @@ class Solver:
     """Factor-once / solve-many GPU SPIKE handle (include/spike_ext.h)."""
 
     def __init__(self, precision="fp64", partitions=0, tol=1e-10, max_refine=8, reduced_maxit=64,
-                 reduced_tol=1e-15, verbose=False, reduced_solver="auto", graph=False):
+                 reduced_tol=1e-15, verbose=False, reduced_solver="auto", graph=False, tune="latency"):
         self._L = load()
         self._opts = _lib.default_opts(precision=_prec_code(precision), partitions=int(partitions), tol=float(tol),
                                        max_refine=int(max_refine), reduced_maxit=int(reduced_maxit),
                                        reduced_tol=float(reduced_tol), verbose=int(bool(verbose)),
-                                       reduced_solver=_red_code(reduced_solver), graph=int(bool(graph)))
+                                       reduced_solver=_red_code(reduced_solver), graph=int(bool(graph)),
+                                       tune=_tune_code(tune))
         h = ctypes.c_void_p()
         check(self._L.spike_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_create")
         self._h = h
@@ -225,10 +239,12 @@ class Solver:
 class Batch:
     """Factor and solve many independent device-resident systems together (include/spike_ext.h)."""
 
-    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, verbose=False, partitions=0, graph=False):
+    def __init__(self, precision="mixed", tol=1e-10, max_refine=8, verbose=False, partitions=0, graph=False,
+                 tune="latency"):
         self._L = load()
         self._opts = _lib.default_opts(precision=_prec_code(precision), tol=float(tol), max_refine=int(max_refine),
-                                       verbose=int(bool(verbose)), partitions=int(partitions), graph=int(bool(graph)))
+                                       verbose=int(bool(verbose)), partitions=int(partitions), graph=int(bool(graph)),
+                                       tune=_tune_code(tune))
         h = ctypes.c_void_p()
         check(self._L.spike_batch_create(ctypes.byref(h), ctypes.byref(self._opts)), "spike_batch_create")
         self._h = h

@@ -36,7 +36,7 @@ class Opts(ctypes.Structure):
         ("precision", ctypes.c_int), ("partitions", ctypes.c_int), ("max_refine", ctypes.c_int),
         ("tol", ctypes.c_double), ("reduced_maxit", ctypes.c_int), ("reduced_tol", ctypes.c_double),
         ("sweep_waves", ctypes.c_int), ("verbose", ctypes.c_int), ("reduced_solver", ctypes.c_int),
-        ("graph", ctypes.c_int),
+        ("graph", ctypes.c_int), ("tune", ctypes.c_int),
     ]
 
 
@@ -120,7 +120,7 @@ EXPORTED = [
     "spike_set_matrix_host", "spike_factor", "spike_solve_dev", "spike_solve_host", "spike_get_stats",
     "spike_get_schedule", "spike_get_tips", "spike_get_reduced_solution", "spike_set_stream",
     "spike_error_string", "spike_last_error_message", "spike_batch_create", "spike_batch_destroy", "spike_batch_solve_dev",
-    "spike_batch_get_stats", "spike_autotune_partitions", "spike_residual_host", "spike_band_solve_host",
+    "spike_batch_get_stats", "spike_autotune_partitions", "spike_autotune_partitions_tuned", "spike_residual_host", "spike_band_solve_host",
     "spike_band_solve_dev_noalloc",
     "spike_gen_banded",
     "spike_gen_dense", "spike_rcm_order", "spike_permute_csr",
@@ -162,6 +162,7 @@ def load():
         "spike_batch_solve_dev": (i32, [vp, i32, i32, ctypes.POINTER(System)]),
         "spike_batch_get_stats": (i32, [vp, ctypes.POINTER(Stats), vp]),
         "spike_autotune_partitions": (i32, [i32, i32, i32, i32, i32, i32]),
+        "spike_autotune_partitions_tuned": (i32, [i32, i32, i32, i32, i32, i32, i32]),
         "spike_residual_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, vp, vp]),
         "spike_band_solve_host": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, vp]),
         "spike_band_solve_dev_noalloc": (i32, [i32, i32, i64, vp, vp, vp, i32, vp, i32, vp, i32, vp, vp]),

@@ -47,7 +47,17 @@ typedef struct {
     int    verbose;        /* 1 = stage timings on stderr                                   */
     int    reduced_solver; /* SPIKE_RED_AUTO (default), SPIKE_RED_JACOBI or SPIKE_RED_DIRECT  */
     int    graph;          /* 1: replay the solve phase from a captured HIP graph (see below)  */
+    int    tune;           /* partition autotuning: SPIKE_TUNE_LATENCY (default) or _THROUGHPUT */
 } spike_opts_t;
+
+/* Partition autotuning (opts.partitions = 0).  LATENCY sizes partitions so that one system fills
+ * the device (about 12 blocks each, capped at the factor workgroups the device runs at once);
+ * THROUGHPUT uses partitions of about 96 blocks, for batches of many systems, where the device
+ * is full anyway and every partition adds spike and reduced-system work.  Either way P depends
+ * only on the system and the options, so a system gets the same bits alone and in a batch solved
+ * with the same options. */
+#define SPIKE_TUNE_LATENCY    0
+#define SPIKE_TUNE_THROUGHPUT 1
 
 /* Graph replay (opts.graph = 1).  The decisions of a solve -- which systems need another
  * refinement pass, which reduced iterations have converged -- are taken on the device, so a
@@ -171,8 +181,10 @@ int  spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_s
 int  spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s,
                            double *max_rel_residual_per_system);
 
-/* Partition count the autotuner picks for one system (pure function of its arguments). */
+/* Partition count the autotuner picks for one system (pure function of its arguments);
+ * spike_autotune_partitions is the LATENCY mode. */
 int  spike_autotune_partitions(int n, int ku, int kl, int nrhs, int dtype, int precision);
+int  spike_autotune_partitions_tuned(int n, int ku, int kl, int nrhs, int dtype, int precision, int tune);
 
 /* Relative residual ||A x - b|| / ||b|| of every column (host arrays, FP64 on the GPU). */
 int  spike_residual_host(int dtype, int n, int64_t nnz, const int *rowptr, const int *colind,

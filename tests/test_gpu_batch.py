@@ -99,3 +99,30 @@ def test_windowed_batch_matches_whole(env, monkeypatch):
     np.testing.assert_array_equal(rel_whole, rel_win)
     assert st_win["max_rel_residual"] <= 1e-10
     win.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_throughput_tuning_matches_alone_and_oracle(env, oracle_mod, prec):
+    """Partitions autotuned for throughput (long partitions, for batches of many systems): each
+    system's bits equal its solve alone with the same tuning, and the oracle's solution."""
+    torch, hs = env
+    systems = make_batch(torch, hs, 16, seed=31, n_hi=60000)
+    bt = hs.Batch(prec, tol=1e-10, tune="throughput")
+    rel, st = bt.solve_dev(systems)
+    torch.cuda.synchronize()
+    assert rel.max() <= 1e-10
+    for s in systems:
+        X = s["x"].cpu().numpy().T
+        X64 = oracle_mod.direct_solve(oracle_mod.CSR(s["A"].n, s["A"].rowptr, s["A"].colind, s["A"].aij), s["B"])
+        assert np.linalg.norm(X - X64) / np.linalg.norm(X64) <= (1e-8 if prec == "mixed" else 1e-11)
+        solo = hs.Solver(prec, tol=1e-10, tune="throughput")
+        solo.set_matrix_dev(s["n"], s["rowptr"], s["colind"], s["aij"])
+        xs = torch.zeros_like(s["x"])
+        solo.solve_dev(s["b"], xs, s["nrhs"])
+        torch.cuda.synchronize()
+        assert torch.equal(xs, s["x"])
+        assert solo.stats()["P"] == hs.autotune_partitions(s["n"], s["k"], s["k"], s["nrhs"],
+                                                           precision=hs.SPIKE_PREC_MIXED if prec == "mixed" else 0,
+                                                           tune="throughput")
+        solo.close()
+    bt.close()

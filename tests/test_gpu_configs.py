@@ -146,7 +146,7 @@ def test_c4_full_batch(hs, oracle_mod):
                             x=torch.zeros((nrhs, n), dtype=torch.float64, device=dev)))
         if i in sample:
             host[i] = (A, B)
-    bt = hs.Batch("mixed", tol=bench.C4["tol"])
+    bt = hs.Batch("mixed", tol=bench.C4["tol"], tune="throughput")        # as bench.py solves it
     rel, st = bt.solve_dev(systems)
     torch.cuda.synchronize()
     assert len(rel) == 4096 and rel.max() <= 1e-10
@@ -157,7 +157,7 @@ def test_c4_full_batch(hs, oracle_mod):
         if n_done < 6:                                         # the same system alone: same bits
             one = [dict(systems[i])]
             one[0]["x"] = torch.zeros_like(systems[i]["x"])
-            hs.Batch("mixed", tol=bench.C4["tol"]).solve_dev(one)
+            hs.Batch("mixed", tol=bench.C4["tol"], tune="throughput").solve_dev(one)
             torch.cuda.synchronize()
             assert torch.equal(one[0]["x"], systems[i]["x"]), i
 

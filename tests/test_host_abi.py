@@ -192,6 +192,10 @@ def test_autotuner_is_pure():
     assert 32 <= a <= 8192
     assert hs.autotune_partitions(1000, 2, 2, 1) >= 1
     assert hs.autotune_partitions(100, 60, 60, 1) == 1
+    # throughput tuning: longer partitions, never more of them than the latency tuning
+    for n, k in ((200_000, 64), (37_000, 4), (1_000, 16)):
+        t = hs.autotune_partitions(n, k, k, 4, tune="throughput")
+        assert 1 <= t <= hs.autotune_partitions(n, k, k, 4)
 
 
 # ---------------------------------------------------------------- drop-in boundary (headers, memory ABI)

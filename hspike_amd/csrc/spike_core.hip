@@ -286,6 +286,7 @@ struct Group {
     DevBuf blkmap;                          // partition of each block
     DevBuf D, U, C, R, Bt, Ct;              // factor type
     DevBuf Wf, Vf;                          // forward-swept W spike (all blocks) and V spike (last two blocks)
+    DevBuf wstop;                           // per partition: blocks of Wf the W sweep wrote (decayed spike)
     DevBuf vt, vb, wt, wb, T;               // H type
     DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
     DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
@@ -415,8 +416,8 @@ struct Engine {
         size_t s = 0;
         for (int gi = 0; gi < ngroups; ++gi) {
             const Group &g = *groups[gi];
-            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb,
-                                  &g.T, &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork, &g.xrm,
+            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt,
+                                  &g.wb, &g.T, &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork, &g.xrm,
                                   &g.rd_ab, &g.rd_x};
             for (auto b : bs) s += b->bytes;
         }
@@ -471,19 +472,19 @@ template <typename T, int NT, int CT, bool VW>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
                   typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, T *Wf, T *Vf,
-                  T *wsave, int ymax, const int *active) {
+                  T *wsave, int ymax, const int *active, int *wstop) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_sweep<T, NT, CT, VW>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
-                           vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active);
+                           vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active, wstop);
 }
 
 template <typename T, int NT, int CT>
 void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *U, const T *Wf,
                     const T *Vf, const typename tr<T>::H *red, const int *part_iface, int ymax, const T *wsave,
-                    int accumulate, const int *active) {
+                    int accumulate, const int *active, const int *wstop) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, U, Wf, Vf, red,
-                           part_iface, ymax, (const Q4<T> *)wsave, accumulate, active);
+                           part_iface, ymax, (const Q4<T> *)wsave, accumulate, active, wstop);
 }
 
 // FP64 residual with TW column lanes per row: the narrowest of 4, 8, 16 covering the chunk's
@@ -686,6 +687,7 @@ int Engine::factor() {
         for (auto b : tb)
             if ((e = b->ensure(g.nblk * bb * ts))) return e;
         if ((e = g.Vf.ensure(np * 2 * bb * ts))) return e;
+        if ((e = g.wstop.ensure(np * sizeof(int)))) return e;
         if ((e = g.Bt.ensure(np * bb * ts)) || (e = g.Ct.ensure(np * bb * ts))) return e;
         DevBuf *hb[] = {&g.vt, &g.vb, &g.wt, &g.wb};
         for (auto b : hb)
@@ -736,9 +738,10 @@ int Engine::factor() {
         HIPCK(hipGetLastError());
     }
     HIPCK(hipEventRecord(ev[5], s));
-    // ---- V / W spike tips (partitions of systems with P > 1)
+    // ---- V / W spike tips (partitions of systems with P > 1); wstop starts at zero (fill kernel)
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
+        if ((e = fill_zero<int>(g.wstop, g.parts.size(), s))) return e;
         if (g.vw_uploaded) {
             if (g.n_items_vw) {
                 T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
@@ -747,7 +750,7 @@ int Engine::factor() {
                                             g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                             g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                             (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
-                                            (const int *)nullptr);
+                                            (const int *)nullptr, g.wstop.as<int>());
                 })));
                 HIPCK(hipGetLastError());
             }
@@ -777,7 +780,7 @@ int Engine::factor() {
                                     g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                     g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                     (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
-                                    (const int *)nullptr);
+                                    (const int *)nullptr, g.wstop.as<int>());
         })));
         HIPCK(hipGetLastError());
     }
@@ -1142,7 +1145,7 @@ std::vector<uint64_t> Engine::graph_key(int ymax) const {
         const Group &g = *groups[gi];
         k.insert(k.end(), {(uint64_t)g.NT, (uint64_t)g.tcode, (uint64_t)g.ct_y, (uint64_t)g.n_items_y, (uint64_t)g.n_items_r,
                            (uint64_t)g.ifs.size(), (uint64_t)g.chunks.size(), (uint64_t)g.sys.size(), (uint64_t)g.ymax});
-        const DevBuf *bufs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
+        const DevBuf *bufs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
                                 &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.partials, &g.norms, &g.conv, &g.rwork,
                                 &g.xrm, &g.d_active, &g.d_red_active, &g.d_sysstat, &g.d_direct, &g.d_summary,
                                 &g.items_yb, &g.items_yr, &g.items_rb, &g.items_rr, &g.d_parts, &g.d_ifs, &g.d_part_iface,
@@ -1240,7 +1243,8 @@ int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, doub
                 (launch_sweep<T, NT, CT, false>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                         g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
                                         (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(),
-                                        (T *)nullptr, (T *)nullptr, g.scratch.as<T>(), g.ymax, g.d_active.as<int>());
+                                        (T *)nullptr, (T *)nullptr, g.scratch.as<T>(), g.ymax, g.d_active.as<int>(),
+                                        (int *)nullptr);
             })));
             HIPCK(hipGetLastError());
         }
@@ -1323,7 +1327,8 @@ int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, doub
                 typedef typename tr<T>::H H;
                 (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.U.as<T>(),
                                           g.Wf.as<T>(), g.Vf.as<T>(), (const H *)g.red0.p, g.d_part_iface.as<int>(),
-                                          g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0, g.d_active.as<int>());
+                                          g.ymax, g.scratch.as<T>(), it > 0 ? 1 : 0, g.d_active.as<int>(),
+                                          (const int *)g.wstop.as<int>());
             })));
             HIPCK(hipGetLastError());
         }

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <type_traits>
 #include <memory>
 #include <thread>
 #include <condition_variable>
@@ -287,6 +288,7 @@ struct Group {
     DevBuf D, U, C, R, Bt, Ct;              // factor type
     DevBuf Wf, Vf;                          // forward-swept W spike (all blocks) and V spike (last two blocks)
     DevBuf wstop;                           // per partition: blocks of Wf the W sweep wrote (decayed spike)
+    DevBuf rstop;                           // per partition: blocks of R the factor formed (decayed R_j)
     DevBuf vt, vb, wt, wb, T;               // H type
     DevBuf yt, yb, red0, red1, scratch, rwork, xrm, partials, norms, conv, pivinfo, status;
     DevBuf items_vw, items_yb, items_yr, items_rb, items_rr;
@@ -416,8 +418,8 @@ struct Engine {
         size_t s = 0;
         for (int gi = 0; gi < ngroups; ++gi) {
             const Group &g = *groups[gi];
-            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt,
-                                  &g.wb, &g.T, &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork, &g.xrm,
+            const DevBuf *bs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.rstop, &g.Bt, &g.Ct, &g.vt, &g.vb,
+                                  &g.wt, &g.wb, &g.T, &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.rwork, &g.xrm,
                                   &g.rd_ab, &g.rd_x};
             for (auto b : bs) s += b->bytes;
         }
@@ -472,10 +474,10 @@ template <typename T, int NT, int CT, bool VW>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
                   typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, T *Wf, T *Vf,
-                  T *wsave, int ymax, const int *active, int *wstop) {
+                  T *wsave, int ymax, const int *active, int *wstop, const int *rstop) {
     if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
         hipLaunchKernelGGL((k_sweep<T, NT, CT, VW>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
-                           vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active, wstop);
+                           vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active, wstop, rstop);
 }
 
 template <typename T, int NT, int CT>
@@ -687,7 +689,7 @@ int Engine::factor() {
         for (auto b : tb)
             if ((e = b->ensure(g.nblk * bb * ts))) return e;
         if ((e = g.Vf.ensure(np * 2 * bb * ts))) return e;
-        if ((e = g.wstop.ensure(np * sizeof(int)))) return e;
+        if ((e = g.wstop.ensure(np * sizeof(int))) || (e = g.rstop.ensure(np * sizeof(int)))) return e;
         if ((e = g.Bt.ensure(np * bb * ts)) || (e = g.Ct.ensure(np * bb * ts))) return e;
         DevBuf *hb[] = {&g.vt, &g.vb, &g.wt, &g.wb};
         for (auto b : hb)
@@ -722,7 +724,7 @@ int Engine::factor() {
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), g.pivinfo.as<double>(),
-                               (const int *)nullptr);
+                               (const int *)nullptr, g.rstop.as<int>());
         }));
         HIPCK(hipGetLastError());
     }
@@ -733,7 +735,7 @@ int Engine::factor() {
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             if constexpr (!factor_forms_r<T, NT>())        // else the factor kernel formed R
                 hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
-                                   (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr);
+                                   (const T *)g.U.as<T>(), g.R.as<T>(), (const int *)nullptr, g.rstop.as<int>());
         }));
         HIPCK(hipGetLastError());
     }
@@ -750,14 +752,15 @@ int Engine::factor() {
                                             g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                             g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                             (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
-                                            (const int *)nullptr, g.wstop.as<int>());
+                                            (const int *)nullptr, g.wstop.as<int>(), (const int *)g.rstop.as<int>());
                 })));
                 HIPCK(hipGetLastError());
             }
             continue;
         }
         std::vector<SweepItem> items;
-        g.ct_vw = ct_max(g.tcode, g.NT);
+        // a spike has at most b = 16 NT columns: NT column tiles cover it (more would multiply zeros)
+        g.ct_vw = std::min(ct_max(g.tcode, g.NT), g.NT);
         const int cpi = 16 * g.ct_vw;
         for (int q = 0; q < (int)g.parts.size(); ++q) {
             const PartDesc &d = g.parts[q];
@@ -780,7 +783,7 @@ int Engine::factor() {
                                     g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
                                     g.Ct.as<T>(), g.vt.as<H>(), g.vb.as<H>(), g.wt.as<H>(), g.wb.as<H>(),
                                     (H *)nullptr, (H *)nullptr, g.Wf.as<T>(), g.Vf.as<T>(), (T *)nullptr, 0,
-                                    (const int *)nullptr, g.wstop.as<int>());
+                                    (const int *)nullptr, g.wstop.as<int>(), (const int *)g.rstop.as<int>());
         })));
         HIPCK(hipGetLastError());
     }
@@ -1145,7 +1148,7 @@ std::vector<uint64_t> Engine::graph_key(int ymax) const {
         const Group &g = *groups[gi];
         k.insert(k.end(), {(uint64_t)g.NT, (uint64_t)g.tcode, (uint64_t)g.ct_y, (uint64_t)g.n_items_y, (uint64_t)g.n_items_r,
                            (uint64_t)g.ifs.size(), (uint64_t)g.chunks.size(), (uint64_t)g.sys.size(), (uint64_t)g.ymax});
-        const DevBuf *bufs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
+        const DevBuf *bufs[] = {&g.D, &g.U, &g.C, &g.R, &g.Wf, &g.Vf, &g.wstop, &g.rstop, &g.Bt, &g.Ct, &g.vt, &g.vb, &g.wt, &g.wb, &g.T,
                                 &g.yt, &g.yb, &g.red0, &g.red1, &g.scratch, &g.partials, &g.norms, &g.conv, &g.rwork,
                                 &g.xrm, &g.d_active, &g.d_red_active, &g.d_sysstat, &g.d_direct, &g.d_summary,
                                 &g.items_yb, &g.items_yr, &g.items_rb, &g.items_rr, &g.d_parts, &g.d_ifs, &g.d_part_iface,
@@ -1244,7 +1247,7 @@ int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, doub
                                         g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
                                         (H *)nullptr, (H *)nullptr, (H *)nullptr, g.yt.as<H>(), g.yb.as<H>(),
                                         (T *)nullptr, (T *)nullptr, g.scratch.as<T>(), g.ymax, g.d_active.as<int>(),
-                                        (int *)nullptr);
+                                        (int *)nullptr, (const int *)g.rstop.as<int>());
             })));
             HIPCK(hipGetLastError());
         }

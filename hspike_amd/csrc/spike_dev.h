@@ -25,6 +25,26 @@ struct cx {
 typedef cx<float> zf;
 typedef cx<double> zd;
 
+// ---------------------------------------------------------------- memory through descriptors
+// Most array pointers reach the kernels inside descriptors in memory (systems, work items); the
+// compiler treats them as generic and accesses them with flat instructions.  Flat operations are
+// counted with the LDS operations as well, so every wait for an LDS access (the exchanges and
+// staging of each chain) also waits for the flat loads issued just before it -- the prefetches
+// of the next block.  These pointers address device or host-mapped memory, never LDS or
+// scratch, so gld / gst access them through global-address-space pointers.
+#define HS_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ T gld(const T *p) { return *(const HS_GLOBAL T *)p; }
+template <typename R> __device__ __forceinline__ cx<R> gld(const cx<R> *p) {
+    const HS_GLOBAL R *q = (const HS_GLOBAL R *)p;
+    return cx<R>{q[0], q[1]};
+}
+template <typename T> __device__ __forceinline__ void gst(T *p, T v) { *(HS_GLOBAL T *)p = v; }
+template <typename R> __device__ __forceinline__ void gst(cx<R> *p, cx<R> v) {
+    HS_GLOBAL R *q = (HS_GLOBAL R *)p;
+    q[0] = v.re;
+    q[1] = v.im;
+}
+
 template <typename T> struct tr;
 template <> struct tr<float>  { typedef float  R; typedef f4v V; static constexpr bool C = false; typedef double H; };
 template <> struct tr<double> { typedef double R; typedef d4v V; static constexpr bool C = false; typedef double H; };

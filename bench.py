@@ -106,7 +106,8 @@ def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
     panels once per pass); implementation overheads never enter."""
     if name == "factor":      # banded LU of all partitions
         return 2.0 * n * kl * ku, 2.0 * n * (kl + ku + 1) * s
-    if name == "spikes":      # V/W spike tips: m_i = ku [i < P-1] + kl [i > 0] columns per partition
+    if name == "spikes":      # V/W spike tips over whole partitions: m_i = ku [i < P-1] + kl [i > 0]
+        # columns per partition (the chains stop early; spike_counts charges the rows they swept)
         m_avg = (ku * (P - 1) + kl * (P - 1)) / max(P, 1)
         return 2.0 * n * (kl + ku) * m_avg, n * (kl + ku + 1) * s
     if name == "recover":     # forward + backward sweep over nrhs columns, f read, x written
@@ -116,6 +117,15 @@ def algorithmic_counts(name, n, nnz, ku, kl, nrhs, P, s):
     if name == "residual":    # FP64 CSR residual of every column
         return 2.0 * nnz * nrhs, nnz * 12.0 + 4.0 * (n + 1) + 3.0 * n * nrhs * 8
     raise KeyError(name)
+
+
+def spike_counts(work, s):
+    """Algorithmic flops and bytes of the V / W spike stage from the rows its chains swept in the
+    last factorization (spike_get_spike_work): a W chain stops once its spike has decayed below
+    2^-70 of its first block and a V chain covers only the blocks its right-hand side reaches, so
+    the stage reads the band of those rows, not of the whole partition."""
+    rows, band, flops = work
+    return flops, band * s
 
 
 def profile_table(config, kind):
@@ -326,13 +336,15 @@ def bench_single(args, torch, hs):
             stats.append(s.stats())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        work = s.spike_work()                          # spike chains of the last factorization
         tr = time.perf_counter()                       # resident factors (repeat-RHS solves)
         for _ in range(max(2, steps // 2)):
             s.solve_dev(d_b, d_x, nrhs)
         torch.cuda.synchronize()
         t_res = (time.perf_counter() - tr) / max(2, steps // 2)
         x = d_x.cpu().numpy().T.copy(order="F")
-        return dict(wall=wall, lat=np.array(lat), stats=stats, last=stats[-1], x=x, resident_ms=t_res * 1e3)
+        return dict(wall=wall, lat=np.array(lat), stats=stats, last=stats[-1], x=x, resident_ms=t_res * 1e3,
+                    work=work)
 
     log("generated %s: n=%d nnz=%d in %.2f s" % (args.config, n, A.nnz, t_gen))
     # timed steps: factor + the solve replayed from its captured graph; per-stage times from
@@ -367,6 +379,7 @@ def bench_single(args, torch, hs):
                   "ms_solve_total")}
     passes = max(1, eager["last"]["solve_passes"])
     counts = {st: algorithmic_counts(st, n, A.nnz, k, k, nrhs, P_auto, 4) for st in STAGES}
+    counts["spikes"] = spike_counts(eager["work"], 4)
     names = kernel_names_for([mixed["last"]["b"] // 16], nrhs)
     stage_roof = stage_roofline(avg, counts, passes, 1, args.config, names)
     roof = roofline_of(stage_roof, avg)
@@ -462,8 +475,9 @@ def bench_batch(args, torch, hs):
             sts.append(st)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        work = bt.spike_work()
         bt.close()
-        return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts)
+        return dict(wall=wall, lat=np.array(lat), rel=rels, stats=sts, work=work)
 
     # timed steps: factor + the solve replayed from its captured graph; per-stage times from
     # eager steps (a replay does not time its stages)
@@ -495,6 +509,7 @@ def bench_batch(args, torch, hs):
             f, b = algorithmic_counts(st, n, z, kk, kk, nrhs, P, 4)
             counts[st][0] += f
             counts[st][1] += b
+    counts["spikes"] = spike_counts(eager["work"], 4)
     names = kernel_names_for(sorted(nts), C4["nrhs_hi"])
     stage_roof = stage_roofline(avg, counts, passes, groups, "c4", names)
     roof = roofline_of(stage_roof, avg)

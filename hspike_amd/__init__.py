@@ -197,6 +197,14 @@ class Solver:
         check(self._L.spike_get_stats(self._h, ctypes.byref(s)), "spike_get_stats")
         return s.as_dict()
 
+    def spike_work(self):
+        """Work of the V / W spike chains in the last factorization: (rows swept, band entries read,
+        flops) -- a W chain stops once its spike has decayed, a V chain starts where its
+        right-hand side is nonzero (spike_get_spike_work)."""
+        w = np.zeros(3)
+        check(self._L.spike_get_spike_work(self._h, _ptr(w)), "spike_get_spike_work")
+        return tuple(float(v) for v in w)
+
     def schedule(self):
         P = ctypes.c_int()
         check(self._L.spike_get_schedule(self._h, ctypes.byref(P), None, 0), "spike_get_schedule")
@@ -279,6 +287,12 @@ class Batch:
         st = _lib.Stats()
         check(self._L.spike_batch_get_stats(self._h, ctypes.byref(st), _ptr(rel)), "spike_batch_get_stats")
         return rel, st.as_dict()
+
+    def spike_work(self):
+        """Solver.spike_work for the batch's last factorization (spike_batch_get_spike_work)."""
+        w = np.zeros(3)
+        check(self._L.spike_batch_get_spike_work(self._h, _ptr(w)), "spike_batch_get_spike_work")
+        return tuple(float(v) for v in w)
 
 
 class Server:

@@ -118,6 +118,7 @@ EXPORTED = [
     # spike_ext.h
     "spike_default_opts", "spike_create", "spike_destroy", "spike_set_opts", "spike_set_matrix_dev",
     "spike_set_matrix_host", "spike_factor", "spike_solve_dev", "spike_solve_host", "spike_get_stats",
+    "spike_get_spike_work", "spike_batch_get_spike_work",
     "spike_get_schedule", "spike_get_tips", "spike_get_reduced_solution", "spike_set_stream",
     "spike_error_string", "spike_last_error_message", "spike_batch_create", "spike_batch_destroy", "spike_batch_solve_dev",
     "spike_batch_get_stats", "spike_autotune_partitions", "spike_autotune_partitions_tuned", "spike_residual_host", "spike_band_solve_host",
@@ -151,6 +152,8 @@ def load():
         "spike_solve_dev": (i32, [vp, i32, vp, i32, vp, i32]),
         "spike_solve_host": (i32, [vp, i32, vp, i32, vp, i32]),
         "spike_get_stats": (i32, [vp, ctypes.POINTER(Stats)]),
+        "spike_get_spike_work": (i32, [vp, vp]),
+        "spike_batch_get_spike_work": (i32, [vp, vp]),
         "spike_get_schedule": (i32, [vp, ctypes.POINTER(i32), vp, i32]),
         "spike_get_tips": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "spike_get_reduced_solution": (i32, [vp, vp]),

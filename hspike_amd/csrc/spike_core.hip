@@ -1468,6 +1468,34 @@ void stage_times(spike_stats_t &dst, const spike_stats_t &src, bool add) {
     for (auto m : f) dst.*m = add ? dst.*m + src.*m : src.*m;
 }
 
+// Rows the V / W chains of the last factorization swept (the W chains' stop blocks read back):
+// a W chain covers blocks [0, wstop), a V chain blocks [j0, nb), j0 the block holding row ni - ku.
+void spike_work(const Engine &E, double *w) {
+    w[0] = w[1] = w[2] = 0.0;
+    if (!E.factored) return;
+    for (int gi = 0; gi < E.ngroups; ++gi) {
+        const Group &g = *E.groups[gi];
+        const size_t np = g.parts.size();
+        if (!np || !g.wstop.p) continue;
+        std::vector<int> ws(np);
+        if (hipStreamSynchronize(E.stream) != hipSuccess ||
+            hipMemcpy(ws.data(), g.wstop.p, np * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            return;
+        for (size_t q = 0; q < np; ++q) {
+            const PartDesc &d = g.parts[q];
+            if (d.P == 1) continue;
+            const bool hw = d.p > 0, hv = d.p < d.P - 1;
+            const int64_t wr = hw ? std::min<int64_t>((int64_t)ws[q] * g.b, d.ni) : 0;
+            const int64_t v0 = hv ? (int64_t)((d.ni - d.ku) / g.b) * g.b : d.ni;
+            const int64_t rows = wr + (d.ni - std::max(v0, wr));
+            const double band = d.kl + d.ku + 1.0, k2 = 2.0 * (d.kl + d.ku);
+            w[0] += (double)rows;
+            w[1] += (double)rows * band;
+            w[2] += k2 * ((double)wr * d.kl + (double)(d.ni - v0) * d.ku);
+        }
+    }
+}
+
 void fill_stats(const Engine &E, spike_stats_t *s) {
     *s = E.st;
     if (!E.sys.empty()) {
@@ -1637,6 +1665,12 @@ int spike_solve_host(spike_handle_t *h, int nrhs, const void *b, int ldb, void *
     if (rc == SPIKE_SUCCESS || rc == SPIKE_ERR_NOT_CONVERGED)
         HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, E.stage_x.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
     return rc;
+}
+
+int spike_get_spike_work(const spike_handle_t *h, double w[3]) {
+    if (!h || !w) return SPIKE_ERR_INVALID_ARG;
+    spike_work(h->E, w);
+    return SPIKE_SUCCESS;
 }
 
 int spike_get_stats(const spike_handle_t *h, spike_stats_t *s) {
@@ -1823,6 +1857,12 @@ int spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_sy
                 nsys, bt->windows, bt->acc.ms_factor_total, bt->acc.ms_solve_total, now_ms() - t0, mr);
     }
     return e;
+}
+
+int spike_batch_get_spike_work(const spike_batch_t *bt, double w[3]) {
+    if (!bt || !w) return SPIKE_ERR_INVALID_ARG;
+    spike_work(bt->E, w);
+    return SPIKE_SUCCESS;
 }
 
 int spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s, double *rel) {

@@ -139,6 +139,14 @@ int  spike_solve_host(spike_handle_t *h, int nrhs, const void *b, int ldb, void 
 
 int  spike_get_stats(const spike_handle_t *h, spike_stats_t *s);
 
+/* Work of the V / W spike chains in the last factorization, for throughput accounting:
+ * w[0] = rows swept (per partition the union of its V and W chains), w[1] = the sum over those
+ * rows of kl + ku + 1 (band entries the stage reads), w[2] = the sum over swept rows of
+ * 2 (kl + ku) x the chain's spike columns.  A W chain stops once its spike has decayed below
+ * 2^-70 of its first block and a V chain starts at the block its right-hand side reaches, so a
+ * partition's chains sweep fewer rows than it holds.  Waits for the handle's stream. */
+int  spike_get_spike_work(const spike_handle_t *h, double w[3]);
+
 /* Schedule of the factored matrix: P and the first row of each partition (rows[P+1]). */
 int  spike_get_schedule(const spike_handle_t *h, int *P, int *rows, int cap);
 
@@ -180,6 +188,9 @@ int  spike_batch_destroy(spike_batch_t *bt);
 int  spike_batch_solve_dev(spike_batch_t *bt, int dtype, int nsys, const spike_system_t *sys);
 int  spike_batch_get_stats(const spike_batch_t *bt, spike_stats_t *s,
                            double *max_rel_residual_per_system);
+/* spike_get_spike_work for the batch's last factorization (its last window of systems when the
+ * batch was solved in windows). */
+int  spike_batch_get_spike_work(const spike_batch_t *bt, double w[3]);
 
 /* Partition count the autotuner picks for one system (pure function of its arguments);
  * spike_autotune_partitions is the LATENCY mode. */

@@ -2,10 +2,12 @@
 //
 // Pipeline per batch of systems (a single-system handle is a batch of one):
 //   validate + bandwidth (one launch over every system's rows)
-//   group systems by block size b = 16 * NT, NT in {1, 2, 4, 8}
+//   group systems by block size b = 16 * NT, NT in {1, 2, 4, 8, 16}
 //   per group: CSR -> blocks, block LU with the top block rows R_j (k_factor; k_rtop where the
 //              R panels do not fit in LDS), V/W spike tips keeping their forward sweeps
 //              (k_sweep), reduced-system setup (k_red_setup_mfma)                 -- spike_factor
+//   (b = 256 and complex FP64 at b = 128: the same stages with the factor state in global
+//   memory -- k_factor_gm, k_sweep_gm, k_recover_gm, k_red_setup_gm)
 //   per RHS chunk: y tips keeping the forward sweep (k_sweep), block-Jacobi reduced solve in
 //              FP64 (k_red_jacobi_q / k_red_jacobi), recovery as one backward chain into the FP64
 //              solution (k_recover), FP64 residual (k_residual); mixed precision repeats on the
@@ -34,6 +36,7 @@
 
 #include "spike_dev.h"
 #include "spike_kernels.inc"
+#include "spike_kernels_wide.inc"
 #include "spike_kernels_aux.inc"
 #include "spike_band.inc"
 #include "spike_ext.h"
@@ -149,8 +152,15 @@ int block_nt(int ku, int kl) {
     if (k <= 32) return 2;
     if (k <= 64) return 4;
     if (k <= 128) return 8;
+    if (k <= 256) return 16;
     return 0;
 }
+
+// Groups whose factor state does not fit in registers run the global-memory kernels of
+// spike_kernels_wide.inc: b = 256, and complex FP64 at b = 128.
+bool wide_group(int tcode, int nt) { return nt == 16 || (tcode == 3 && nt == 8); }
+// rows per conversion slab: three 16-row slabs of complex FP64 at b = 256 exceed the LDS
+constexpr int conv_rows(int tbytes, int nt) { return (tbytes == 16 && nt == 16) ? 8 : 16; }
 
 // tcode: 0 float, 1 double, 2 complex float, 3 complex double
 int type_code(int dtype, int precision) { return (dtype == SPIKE_DTYPE_Z ? 2 : 0) + (precision == SPIKE_PREC_FP64 ? 1 : 0); }
@@ -170,6 +180,36 @@ double now_ms() {
     case 4: { constexpr int NT = 4; BODY; } break;           \
     case 8: { constexpr int NT = 8; BODY; } break;           \
     default: break;                                          \
+    }
+
+// every block size; the launch helpers route the wide groups to the global-memory kernels
+#define NTA_SWITCH(NTV, BODY)                                \
+    switch (NTV) {                                           \
+    case 1: { constexpr int NT = 1; BODY; } break;           \
+    case 2: { constexpr int NT = 2; BODY; } break;           \
+    case 4: { constexpr int NT = 4; BODY; } break;           \
+    case 8: { constexpr int NT = 8; BODY; } break;           \
+    case 16: { constexpr int NT = 16; BODY; } break;         \
+    default: break;                                          \
+    }
+
+// the wide groups: (T, NT) pairs of wide_group()
+#define WIDE_SWITCH(TC, NTV, BODY)                                              \
+    {                                                                           \
+    if ((NTV) == 16) {                                                          \
+        constexpr int NT = 16;                                                  \
+        switch (TC) {                                                           \
+        case 0: { typedef float T; BODY; } break;                               \
+        case 1: { typedef double T; BODY; } break;                              \
+        case 2: { typedef zf T; BODY; } break;                                  \
+        case 3: { typedef zd T; BODY; } break;                                  \
+        default: break;                                                         \
+        }                                                                       \
+    } else if ((TC) == 3 && (NTV) == 8) {                                       \
+        constexpr int NT = 8;                                                   \
+        typedef zd T;                                                           \
+        BODY;                                                                   \
+    }                                                                           \
     }
 
 #define T_SWITCH(TC, BODY)                                   \
@@ -195,7 +235,8 @@ int factor_slots(int tcode, int nt) {
     if (it != cache.end()) return it->second;
     int dev = 0, cus = 0, per = 0;
     const void *fn = nullptr;
-    T_SWITCH(tcode, NT_SWITCH(nt, { fn = (const void *)(k_factor<T, NT>); }));
+    if (wide_group(tcode, nt)) WIDE_SWITCH(tcode, nt, { fn = (const void *)(k_factor_gm<T, NT>); })
+    else T_SWITCH(tcode, NT_SWITCH(nt, { fn = (const void *)(k_factor<T, NT>); }));
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         !fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt * 64, 0) != hipSuccess)
         cus = per = 0;
@@ -470,12 +511,19 @@ int ct_max(int tcode, int NT) {
 }
 int ct_for(int ncols, int ctmax) { return std::min(ctmax, ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 4)); }
 
+// wide groups (wide_group(): b = 256, complex FP64 at b = 128) sweep one column tile per item
+template <typename T, int NT> constexpr bool wide_kernels() { return NT == 16 || (sizeof(T) == 16 && NT == 8); }
+
 template <typename T, int NT, int CT, bool VW>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,
                   const T *U, const T *R, const T *Bt, const T *Ct, typename tr<T>::H *vt, typename tr<T>::H *vb,
                   typename tr<T>::H *wt, typename tr<T>::H *wb, typename tr<T>::H *yt, typename tr<T>::H *yb, T *Wf, T *Vf,
                   T *wsave, int ymax, const int *active, int *wstop, const int *rstop) {
-    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
+    if constexpr (wide_kernels<T, NT>()) {
+        if constexpr (CT == 1)
+            hipLaunchKernelGGL((k_sweep_gm<T, NT, VW>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct,
+                               vt, vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active, wstop, rstop);
+    } else if constexpr (sweep_fits((int)sizeof(T), NT, CT))
         hipLaunchKernelGGL((k_sweep<T, NT, CT, VW>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, D, C, U, R, Bt, Ct, vt,
                            vb, wt, wb, yt, yb, Wf, Vf, (Q4<T> *)wsave, ymax, active, wstop, rstop);
 }
@@ -484,7 +532,11 @@ template <typename T, int NT, int CT>
 void launch_recover(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *U, const T *Wf,
                     const T *Vf, const typename tr<T>::H *red, const int *part_iface, int ymax, const T *wsave,
                     int accumulate, const int *active, const int *wstop) {
-    if constexpr (sweep_fits((int)sizeof(T), NT, CT) && !(sizeof(T) == 16 && NT > 4))
+    if constexpr (wide_kernels<T, NT>()) {
+        if constexpr (CT == 1)
+            hipLaunchKernelGGL((k_recover_gm<T, NT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, U, Wf, Vf, red,
+                               part_iface, ymax, (const Q4<T> *)wsave, accumulate, active, wstop);
+    } else if constexpr (sweep_fits((int)sizeof(T), NT, CT))
         hipLaunchKernelGGL((k_recover<T, NT, CT>), dim3(nitems), dim3(NT * 64), 0, s, items, parts, U, Wf, Vf, red,
                            part_iface, ymax, (const Q4<T> *)wsave, accumulate, active, wstop);
 }
@@ -578,7 +630,7 @@ int Engine::factor() {
             S.ku = bw[3 * i];
             S.kl = bw[3 * i + 1];
             S.NT = block_nt(S.ku, S.kl);
-            if (S.NT == 0 || (dtype == SPIKE_DTYPE_Z && S.NT > 4 && o.precision == SPIKE_PREC_FP64)) {
+            if (S.NT == 0) {
                 set_err("bandwidth %ld exceeds this build's limit", (long)std::max(S.ku, S.kl));
                 return SPIKE_ERR_UNSUPPORTED;
             }
@@ -600,9 +652,9 @@ int Engine::factor() {
     // ---- group systems by block size
     const int tcode = type_code(dtype, o.precision);
     const size_t ts = type_bytes(tcode), hs = h_bytes(dtype);
-    int nts[4] = {1, 2, 4, 8};
+    int nts[5] = {1, 2, 4, 8, 16};
     ngroups = 0;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 5; ++q) {
         std::vector<int> members;
         for (int i = 0; i < nsys; ++i)
             if (sys[i].NT == nts[q]) members.push_back(i);
@@ -708,12 +760,19 @@ int Engine::factor() {
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         const int np = (int)g.parts.size();
-        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
-            hipLaunchKernelGGL((k_csr_to_blocks<T, NT>), dim3((unsigned)(g.nblk * NT)), dim3(256), 0, s,
-                               g.blkmap.as<int>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(),
-                               g.D.as<T>(), g.U.as<T>(), g.C.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(),
-                               g.status.as<int>());
-        }));
+        // the wide groups' factor reads B_j from the Wf array and writes -Uq_j to its own block
+        const bool wide = wide_group(g.tcode, g.NT);
+#define CONV_LAUNCH                                                                                            \
+    {                                                                                                          \
+        constexpr int RS = conv_rows((int)sizeof(T), NT);                                                      \
+        hipLaunchKernelGGL((k_csr_to_blocks<T, NT, RS>), dim3((unsigned)(g.nblk * (16 * NT / RS))), dim3(256), 0, s, \
+                           g.blkmap.as<int>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),  \
+                           wide ? g.Wf.as<T>() : g.U.as<T>(), g.C.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(),         \
+                           g.status.as<int>());                                                                \
+    }
+        if (g.NT == 16) WIDE_SWITCH(g.tcode, 16, CONV_LAUNCH)
+        else T_SWITCH(g.tcode, NT_SWITCH(g.NT, CONV_LAUNCH));
+#undef CONV_LAUNCH
         HIPCK(hipGetLastError());
         (void)np;
     }
@@ -721,17 +780,26 @@ int Engine::factor() {
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         const int np = (int)g.parts.size();
-        T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
-            hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
-                               g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), g.pivinfo.as<double>(),
-                               (const int *)nullptr, g.rstop.as<int>());
-        }));
+        if (wide_group(g.tcode, g.NT)) {
+            WIDE_SWITCH(g.tcode, g.NT, {
+                hipLaunchKernelGGL((k_factor_gm<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
+                                   g.D.as<T>(), g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), (const T *)g.Wf.as<T>(),
+                                   g.pivinfo.as<double>(), g.rstop.as<int>());
+            });
+        } else {
+            T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
+                hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
+                                   g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), g.pivinfo.as<double>(),
+                                   (const int *)nullptr, g.rstop.as<int>());
+            }));
+        }
         HIPCK(hipGetLastError());
     }
     HIPCK(hipEventRecord(ev[4], s));
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         const int np = (int)g.parts.size();
+        if (wide_group(g.tcode, g.NT)) continue;          // k_factor_gm forms R
         T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
             if constexpr (!factor_forms_r<T, NT>())        // else the factor kernel formed R
                 hipLaunchKernelGGL((k_rtop<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
@@ -746,7 +814,7 @@ int Engine::factor() {
         if ((e = fill_zero<int>(g.wstop, g.parts.size(), s))) return e;
         if (g.vw_uploaded) {
             if (g.n_items_vw) {
-                T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
+                T_SWITCH(g.tcode, NTA_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
                     typedef typename tr<T>::H H;
                     (launch_sweep<T, NT, CT, true>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                             g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
@@ -760,7 +828,7 @@ int Engine::factor() {
         }
         std::vector<SweepItem> items;
         // a spike has at most b = 16 NT columns: NT column tiles cover it (more would multiply zeros)
-        g.ct_vw = std::min(ct_max(g.tcode, g.NT), g.NT);
+        g.ct_vw = wide_group(g.tcode, g.NT) ? 1 : std::min(ct_max(g.tcode, g.NT), g.NT);
         const int cpi = 16 * g.ct_vw;
         for (int q = 0; q < (int)g.parts.size(); ++q) {
             const PartDesc &d = g.parts[q];
@@ -777,7 +845,7 @@ int Engine::factor() {
         g.vw_uploaded = true;
         if (items.empty()) continue;
         if ((e = upload(g.items_vw, items.data(), items.size() * sizeof(SweepItem), s))) return e;
-        T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
+        T_SWITCH(g.tcode, NTA_SWITCH(g.NT, CT_SWITCH(g.ct_vw, {
             typedef typename tr<T>::H H;
             (launch_sweep<T, NT, CT, true>)(g.n_items_vw, s, g.items_vw.as<SweepItem>(), g.d_parts.as<PartDesc>(),
                                     g.D.as<T>(), g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(),
@@ -792,7 +860,16 @@ int Engine::factor() {
     for (int gi = 0; gi < ngroups; ++gi) {
         Group &g = *groups[gi];
         if (g.ifs.empty()) continue;
-        if (dtype == SPIKE_DTYPE_Z) {
+        if (g.NT == 16) {                                   // global-memory inversion (16-wide panels)
+            if (dtype == SPIKE_DTYPE_Z)
+                hipLaunchKernelGGL((k_red_setup_gm<zd, 16>), dim3((unsigned)g.ifs.size()), dim3(16 * 64), 0, s,
+                                   g.d_ifs.as<IfaceDesc>(), (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(),
+                                   g.status.as<int>() + 1, (const int *)nullptr);
+            else
+                hipLaunchKernelGGL((k_red_setup_gm<double, 16>), dim3((unsigned)g.ifs.size()), dim3(16 * 64), 0, s,
+                                   g.d_ifs.as<IfaceDesc>(), (const double *)g.vb.p, (const double *)g.wt.p,
+                                   g.T.as<double>(), g.status.as<int>() + 1, (const int *)nullptr);
+        } else if (dtype == SPIKE_DTYPE_Z) {
             NT_SWITCH(g.NT, {
                 hipLaunchKernelGGL((k_red_setup_mfma<zd, NT>), dim3((unsigned)g.ifs.size()), dim3(NT * 64), 0, s,
                                    g.d_ifs.as<IfaceDesc>(), (const zd *)g.vb.p, (const zd *)g.wt.p, g.T.as<zd>(),
@@ -821,9 +898,10 @@ int Engine::factor() {
                 set_err("zero or non-finite pivot in partition %s%ld", "", (long)q);
                 return SPIKE_ERR_SINGULAR;
             }
-        if (stv[1] && !g.ifs.empty()) {
-            // a 4x4 panel pivot of the MFMA inversion was zero: redo T_i with full partial pivoting
-            const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
+        // a 4x4 panel pivot of the MFMA inversion was zero: redo T_i with full partial pivoting
+        // (the whole block in LDS; b = 256 pivots inside 16 x 16 panels already)
+        const size_t smem = ((size_t)g.b * g.b + 2 * g.b) * hs;
+        if (stv[1] && !g.ifs.empty() && smem <= 160 * 1024) {
             if ((e = zero_ints(g.status, 4))) return e;
             if (dtype == SPIKE_DTYPE_Z) {
                 HIPCK(hipFuncSetAttribute((const void *)k_red_setup<zd>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
@@ -1053,7 +1131,7 @@ int Engine::solve_setup(int c0, int ymax, bool &cached_all) {
         cached_all = false;
         if ((e = upload(g.d_sysdev, g.sysdev.data(), ngs * sizeof(SysDev), s))) return e;
         // items: Y sweeps and recoveries, from b (first pass) and from r (refinement passes)
-        g.ct_y = ct_for(ymax, ct_max(g.tcode, g.NT));
+        g.ct_y = wide_group(g.tcode, g.NT) ? 1 : ct_for(ymax, ct_max(g.tcode, g.NT));
         const int cpi = 16 * g.ct_y;
         std::vector<SweepItem> yb_, yr_, rb_, rr_;
         for (int q = 0; q < (int)np; ++q) {
@@ -1241,7 +1319,7 @@ int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, doub
             Group &g = *groups[gi];
             if (!g.n_items_y) continue;
             const DevBuf &items = it == 0 ? g.items_yb : g.items_yr;
-            T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
+            T_SWITCH(g.tcode, NTA_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
                 (launch_sweep<T, NT, CT, false>)(g.n_items_y, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),
                                         g.C.as<T>(), g.U.as<T>(), g.R.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(), (H *)nullptr,
@@ -1326,7 +1404,7 @@ int Engine::run_passes(int ymax, bool capture, double rtol, int max_refine, doub
             Group &g = *groups[gi];
             if (!g.n_items_r) continue;
             const DevBuf &items = it == 0 ? g.items_rb : g.items_rr;
-            T_SWITCH(g.tcode, NT_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
+            T_SWITCH(g.tcode, NTA_SWITCH(g.NT, CT_SWITCH(g.ct_y, {
                 typedef typename tr<T>::H H;
                 (launch_recover<T, NT, CT>)(g.n_items_r, s, items.as<SweepItem>(), g.d_parts.as<PartDesc>(), g.U.as<T>(),
                                           g.Wf.as<T>(), g.Vf.as<T>(), (const H *)g.red0.p, g.d_part_iface.as<int>(),
@@ -1390,9 +1468,21 @@ int Engine::direct_pass(Group &g, int ymax) {
 int Engine::launch_jacobi(Group &g, int ncc, int ymax, const DevBuf &xo, DevBuf &xn) {
     hipStream_t s = stream;
     const unsigned nb = (unsigned)(g.ifs.size() * ncc);
-    if (dtype == SPIKE_DTYPE_Z) {
+    if (g.NT == 16) {               // column sub-chunks keep three b x CK vectors within the LDS
+        if (dtype == SPIKE_DTYPE_Z)
+            hipLaunchKernelGGL((k_red_jacobi<zd, 16, 8>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
+                               (const zd *)g.vt.p, (const zd *)g.vb.p, (const zd *)g.wt.p, (const zd *)g.wb.p,
+                               (const zd *)g.yt.p, (const zd *)g.yb.p, (const zd *)g.T.p, (const zd *)xo.p, xn.as<zd>(),
+                               g.conv.as<double>(), ymax, ymax, g.d_red_active.as<int>());
+        else
+            hipLaunchKernelGGL((k_red_jacobi<double, 16, 16>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
+                               (const double *)g.vt.p, (const double *)g.vb.p, (const double *)g.wt.p,
+                               (const double *)g.wb.p, (const double *)g.yt.p, (const double *)g.yb.p,
+                               (const double *)g.T.p, (const double *)xo.p, xn.as<double>(), g.conv.as<double>(), ymax,
+                               ymax, g.d_red_active.as<int>());
+    } else if (dtype == SPIKE_DTYPE_Z) {
         NT_SWITCH(g.NT, {
-            hipLaunchKernelGGL((k_red_jacobi<zd, NT>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p,
+            hipLaunchKernelGGL((k_red_jacobi<zd, NT, 16>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc, (const zd *)g.vt.p,
                                (const zd *)g.vb.p, (const zd *)g.wt.p, (const zd *)g.wb.p, (const zd *)g.yt.p, (const zd *)g.yb.p,
                                (const zd *)g.T.p, (const zd *)xo.p, xn.as<zd>(), g.conv.as<double>(), ymax, ymax,
                                g.d_red_active.as<int>());
@@ -1406,7 +1496,7 @@ int Engine::launch_jacobi(Group &g, int ncc, int ymax, const DevBuf &xo, DevBuf 
                                    (const double *)g.T.p, (const double *)xo.p, xn.as<double>(), g.conv.as<double>(), ymax,
                                    ymax, g.d_red_active.as<int>());
             else
-                hipLaunchKernelGGL((k_red_jacobi<double, NT>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
+                hipLaunchKernelGGL((k_red_jacobi<double, NT, 16>), dim3(nb), dim3(256), 0, s, g.d_ifs.as<IfaceDesc>(), ncc,
                                    (const double *)g.vt.p, (const double *)g.vb.p, (const double *)g.wt.p,
                                    (const double *)g.wb.p, (const double *)g.yt.p, (const double *)g.yb.p,
                                    (const double *)g.T.p, (const double *)xo.p, xn.as<double>(), g.conv.as<double>(), ymax,

@@ -133,8 +133,8 @@ def test_singular_pivot_reported(hs):
     s.close()
 
 
-def test_bandwidth_above_128_unsupported(hs):
-    A = hs.gen_banded(3000, 200, 200, fill=0.05, dom=(1.1, 1.1), diag_floor=1.0, seed=3)
+def test_bandwidth_above_256_unsupported(hs):
+    A = hs.gen_banded(3000, 300, 300, fill=0.05, dom=(1.1, 1.1), diag_floor=1.0, seed=3)
     s = hs.Solver("mixed")
     s.set_matrix(A)
     with pytest.raises(hs.SpikeError) as ex:

@@ -15,7 +15,7 @@ Workloads (synthetic, seeded):
                refinement to 1e-12
   c1           n = 10^4 pentadiagonal, a_ii = sum|a_ij| + 1, 4 partitions, 1 RHS
   c5           request serving: Poisson arrivals (rate --rate), 80% real / 20% complex128, n
-               log-uniform [10^4, 5*10^6], half-bandwidth 2..128 (complex 2..64), 30% fill,
+               log-uniform [10^4, 5*10^6], half-bandwidth 2..256, 30% fill,
                dominance 1.2, nrhs 1..64, 30% of requests repeat a resident matrix with a new RHS
 The CPU baseline is the oracle (oracle/, a C restatement of the reference's SPIKE flow with a
 banded partial-pivot LU in place of SuperLU_MT), timed on the host cores on a bounded sample of
@@ -58,7 +58,7 @@ CONFIGS = {
 SEED_A, SEED_B = 20261202, 7
 C4 = dict(nsys=4096, n_lo=1e3, n_hi=2e5, ks=(4, 8, 16, 32, 64), fill=0.30, dom=1.1, nrhs_hi=8, tol=1e-10,
           seed=20261204)
-C5 = dict(nreq=120, rate=40.0, frac_complex=0.2, n_lo=1e4, n_hi=5e6, k_lo=2, k_hi=128, k_hi_complex=64,
+C5 = dict(nreq=120, rate=40.0, frac_complex=0.2, n_lo=1e4, n_hi=5e6, k_lo=2, k_hi=256, k_hi_complex=256,
           nrhs_hi=64, repeat=0.3, fill=0.30, dom=1.2, tol=1e-10, seed=20261205)
 
 # stage -> stats key of its time (the library's HIP events around the stage's launches)
@@ -642,7 +642,7 @@ def bench_serve(args, hs):
         "ms_per_step": round(mixed["span"] * 1e3 / len(reqs), 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "C5: Poisson request stream, %.0f req/s, 80%% real / 20%% complex128, n log-uniform "
-                               "[1e4, 5e6], half-bandwidth 2-128 (complex 2-64), nrhs 1-64, 30%% repeat-RHS" % args.rate,
+                               "[1e4, 5e6], half-bandwidth 2-256, nrhs 1-64, 30%% repeat-RHS" % args.rate,
                    "requests": len(reqs), "distinct_matrices": len(mats), "nnz_total": nnz, "nrhs_total": rhs,
                    "arrival_rate_per_s": args.rate, "fill": C5["fill"], "seed": C5["seed"], "workers": 3, "max_batch": 8,
                    "precision": "mixed: fp32 factor/spikes, fp64 residual/reduced/update"},

@@ -163,7 +163,7 @@ def test_c4_full_batch(hs, oracle_mod):
 
 
 def test_c5_stream_vs_oracle_and_alone(hs, oracle_mod):
-    reqs, mats = bench.c5_stream(30, bench.C5["seed"], n_hi=1.5e5)
+    reqs, mats = bench.c5_stream(30, bench.C5["seed"], n_hi=8e4)
     A = [hs.gen_banded(n, k, k, fill=bench.C5["fill"], dom=(bench.C5["dom"], bench.C5["dom"]), diag_floor=1.0, seed=sd,
                        complex=cplx) for (cplx, n, k, sd) in mats]
     Bs = [hs.gen_dense(A[mid].n, nrhs, seed=sb, complex=mats[mid][0]) for (_, mid, nrhs, sb) in reqs]

@@ -159,8 +159,10 @@ int block_nt(int ku, int kl) {
 // Groups whose factor state does not fit in registers run the global-memory kernels of
 // spike_kernels_wide.inc: b = 256, and complex FP64 at b = 128.
 bool wide_group(int tcode, int nt) { return nt == 16 || (tcode == 3 && nt == 8); }
-// rows per conversion slab: three 16-row slabs of complex FP64 at b = 256 exceed the LDS
-constexpr int conv_rows(int tbytes, int nt) { return (tbytes == 16 && nt == 16) ? 8 : 16; }
+// block rows per conversion workgroup: 64 for the narrow bands (several whole blocks, so that a
+// workgroup has work enough to amortize its launch), otherwise a 16-row slab of one block (8 for
+// complex FP64 at b = 256, where three 16-row slabs exceed the LDS)
+constexpr int conv_rows(int tbytes, int nt) { return nt <= 2 ? 64 : ((tbytes == 16 && nt == 16) ? 8 : 16); }
 
 // tcode: 0 float, 1 double, 2 complex float, 3 complex double
 int type_code(int dtype, int precision) { return (dtype == SPIKE_DTYPE_Z ? 2 : 0) + (precision == SPIKE_PREC_FP64 ? 1 : 0); }
@@ -765,8 +767,8 @@ int Engine::factor() {
 #define CONV_LAUNCH                                                                                            \
     {                                                                                                          \
         constexpr int RS = conv_rows((int)sizeof(T), NT);                                                      \
-        hipLaunchKernelGGL((k_csr_to_blocks<T, NT, RS>), dim3((unsigned)(g.nblk * (16 * NT / RS))), dim3(256), 0, s, \
-                           g.blkmap.as<int>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(), g.D.as<T>(),  \
+        hipLaunchKernelGGL((k_csr_to_blocks<T, NT, RS>), dim3((unsigned)((g.nblk * 16 * NT + RS - 1) / RS)), dim3(256), 0, s, \
+                           g.blkmap.as<int>(), g.d_sysdev.as<SysDev>(), g.d_parts.as<PartDesc>(), (int64_t)g.nblk, g.D.as<T>(), \
                            wide ? g.Wf.as<T>() : g.U.as<T>(), g.C.as<T>(), g.Bt.as<T>(), g.Ct.as<T>(),         \
                            g.status.as<int>());                                                                \
     }

@@ -253,8 +253,9 @@ def kernel_names_for(nts, nrhs_max):
     RHS columns per chunk (column tiles per sweep workgroup CT with CT * NT <= 16); several block
     sizes are joined with '+'."""
     m = min(nrhs_max, 64)
-    ct_vw = lambda nt: min(4, 16 // nt)
-    ct_y = lambda nt: min(ct_vw(nt), 1 if m <= 16 else (2 if m <= 32 else 4))
+    ct_max = lambda nt: 4 if 4 * nt * 4 <= 64 else (2 if 2 * nt * 4 <= 64 else 1)   # FP32 sweep tiles
+    ct_vw = lambda nt: min(ct_max(nt), nt)           # a spike has at most b = 16 NT columns
+    ct_y = lambda nt: min(ct_max(nt), 1 if m <= 16 else (2 if m <= 32 else 4))
     need = (m + 1) // 2
     tw = 4 if need <= 4 else (8 if need <= 8 else 16)
     return {"factor": "+".join("k_factor<float,%d>" % nt for nt in nts),

@@ -1086,10 +1086,17 @@ int Engine::solve_setup(int c0, int ymax, bool &cached_all) {
             (e = g.norms.ensure(ngs * ymax * 2 * sizeof(double))) ||
             (e = g.conv.ensure(std::max<size_t>(nif, 1) * ceil_div(ymax, 16) * 2 * sizeof(double))))
             return e;
-        size_t rrows = 0;
-        for (size_t gs = 0; gs < ngs; ++gs) rrows += sys[g.sys[gs]].n;
-        const int ldrm = (ymax + 1) & ~1;      // even: a lane's column pair is one aligned load
-        if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(rrows * ldrm * hs))) return e;
+        // row-major copy of each system's iterate (the residual's x gathers): a row stride of the
+        // system's own columns in this chunk, rounded up to even (a lane's column pair is one
+        // aligned load), so a system with few columns gathers from densely packed rows
+        size_t rrows = 0, xrows = 0;
+        auto sys_ldrm = [&](const SysHost &S) { return (std::max(1, std::min(ymax, S.nrhs - c0)) + 1) & ~1; };
+        for (size_t gs = 0; gs < ngs; ++gs) {
+            const SysHost &S = sys[g.sys[gs]];
+            rrows += S.n;
+            xrows += (size_t)S.n * sys_ldrm(S);
+        }
+        if ((e = g.rwork.ensure(rrows * ymax * hs)) || (e = g.xrm.ensure(xrows * hs))) return e;
         std::vector<uint64_t> key = {(uint64_t)c0, (uint64_t)ymax, (uint64_t)(uintptr_t)g.rwork.p,
                                      (uint64_t)(uintptr_t)g.xrm.p};
         for (size_t gs = 0; gs < ngs; ++gs) {
@@ -1098,19 +1105,21 @@ int Engine::solve_setup(int c0, int ymax, bool &cached_all) {
                                    (uint64_t)S.ldx, (uint64_t)S.nrhs});
         }
         const bool cached = key == g.solve_key;
-        size_t roff = 0;
+        size_t roff = 0, xoff = 0;
         for (size_t gs = 0; gs < ngs; ++gs) {
             SysHost &S = sys[g.sys[gs]];
             SysDev &sd = g.sysdev[gs];
+            const int ldrm = sys_ldrm(S);
             const int m = std::max(0, std::min(ymax, S.nrhs - c0));
             sd.m = m;
             sd.ldb = S.ldb; sd.ldx = S.ldx; sd.ldr = S.n;
             sd.bptr = (const char *)S.d_b + (size_t)c0 * S.ldb * hs;
             sd.xptr = (char *)S.d_x + (size_t)c0 * S.ldx * hs;
             sd.rptr = (char *)g.rwork.p + roff * ymax * hs;
-            sd.xrm = (char *)g.xrm.p + roff * ldrm * hs;
+            sd.xrm = (char *)g.xrm.p + xoff * hs;
             sd.ldrm = ldrm;
             roff += S.n;
+            xoff += (size_t)S.n * ldrm;
             S.refine_iters = 0;
         }
         if ((e = g.d_active.ensure(ngs * sizeof(int))) || (e = g.d_red_active.ensure(ngs * sizeof(int))) ||

@@ -789,10 +789,19 @@ int Engine::factor() {
                                    g.pivinfo.as<double>(), g.rstop.as<int>());
             });
         } else {
+            // one copy of R's rows where it adds workgroups and the launch has more partitions than
+            // the device runs at once with two
             T_SWITCH(g.tcode, NT_SWITCH(g.NT, {
-                hipLaunchKernelGGL((k_factor<T, NT>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(), g.D.as<T>(),
-                                   g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(), g.pivinfo.as<double>(),
-                                   (const int *)nullptr, g.rstop.as<int>());
+                bool one = false;
+                if constexpr (factor_one_copy<T, NT>()) one = np > factor_slots(g.tcode, NT);
+                if (one)
+                    hipLaunchKernelGGL((k_factor<T, NT, factor_one_copy<T, NT>() ? 1 : 2>), dim3(np), dim3(NT * 64), 0, s,
+                                       g.d_parts.as<PartDesc>(), g.D.as<T>(), g.U.as<T>(), (const T *)g.C.as<T>(),
+                                       g.R.as<T>(), g.pivinfo.as<double>(), (const int *)nullptr, g.rstop.as<int>());
+                else
+                    hipLaunchKernelGGL((k_factor<T, NT, 2>), dim3(np), dim3(NT * 64), 0, s, g.d_parts.as<PartDesc>(),
+                                       g.D.as<T>(), g.U.as<T>(), (const T *)g.C.as<T>(), g.R.as<T>(),
+                                       g.pivinfo.as<double>(), (const int *)nullptr, g.rstop.as<int>());
             }));
         }
         HIPCK(hipGetLastError());

@@ -126,3 +126,43 @@ def test_throughput_tuning_matches_alone_and_oracle(env, oracle_mod, prec):
                                                            tune="throughput")
         solo.close()
     bt.close()
+
+
+def test_oversubscribed_b64_factor_matches_alone(env, oracle_mod):
+    """A mixed-precision batch of half-bandwidth-64 systems (b = 64) whose partitions outnumber the
+    factor workgroups the device runs at once takes the factor variant that keeps one LDS copy of
+    R's top rows; each system alone (fewer partitions than that) takes the two-copy variant.  Both
+    form the same products, so the solutions are bit-for-bit equal; the first is also checked
+    against the oracle's FP64 direct solve (||x - x64|| / ||x64|| <= 1e-8)."""
+    torch, hs = env
+    dev = torch.device("cuda:0")
+    n, k = 300000, 64
+    cap = hs.autotune_partitions(10 ** 8, k, k, precision=hs.SPIKE_PREC_MIXED)    # one round of the factor
+    p1 = hs.autotune_partitions(n, k, k, precision=hs.SPIKE_PREC_MIXED)
+    nsys = cap // p1 + 1
+    assert nsys * p1 > cap and p1 < cap
+    systems = []
+    for i in range(nsys):
+        A = hs.gen_banded(n, k, k, fill=0.2, dom=(1.1, 1.1), diag_floor=1.0, seed=7100 + i)
+        B = hs.gen_dense(n, 2, seed=7200 + i)
+        systems.append(dict(n=n, nrhs=2, A=A, B=B, rowptr=torch.from_numpy(A.rowptr).to(dev),
+                            colind=torch.from_numpy(A.colind).to(dev), aij=torch.from_numpy(A.aij).to(dev),
+                            b=torch.from_numpy(np.ascontiguousarray(B.T)).to(dev),
+                            x=torch.zeros((2, n), dtype=torch.float64, device=dev)))
+    bt = hs.Batch("mixed", tol=1e-10)
+    rel, st = bt.solve_dev(systems)
+    torch.cuda.synchronize()
+    assert rel.max() <= 1e-10, rel.max()
+    bt.close()
+    s = systems[0]
+    X64 = oracle_mod.direct_solve(oracle_mod.CSR(s["A"].n, s["A"].rowptr, s["A"].colind, s["A"].aij), s["B"])
+    X = s["x"].cpu().numpy().T
+    assert np.linalg.norm(X - X64) / np.linalg.norm(X64) <= 1e-8
+    for s in systems:
+        solo = hs.Solver("mixed", tol=1e-10)
+        solo.set_matrix_dev(s["n"], s["rowptr"], s["colind"], s["aij"])
+        xs = torch.zeros_like(s["x"])
+        solo.solve_dev(s["b"], xs, s["nrhs"])
+        torch.cuda.synchronize()
+        assert torch.equal(xs, s["x"])
+        solo.close()

@@ -248,17 +248,18 @@ def roofline_of(stage_roof, avg_ms):
     return roof
 
 
-def kernel_names_for(nts, nrhs_max):
+def kernel_names_for(nts, nrhs_max, one_copy=()):
     """Instantiations the library launches for block sizes 16*NT (FP32 factors) and up to nrhs_max
     RHS columns per chunk (column tiles per sweep workgroup CT with CT * NT <= 16); several block
-    sizes are joined with '+'."""
+    sizes are joined with '+'.  `one_copy`: block sizes whose factor launch has more partitions than
+    the device runs at once, so it takes the variant with one LDS copy of R's rows (FP32 b = 64)."""
     m = min(nrhs_max, 64)
     ct_max = lambda nt: 4 if 4 * nt * 4 <= 64 else (2 if 2 * nt * 4 <= 64 else 1)   # FP32 sweep tiles
     ct_vw = lambda nt: min(ct_max(nt), nt)           # a spike has at most b = 16 NT columns
     ct_y = lambda nt: min(ct_max(nt), 1 if m <= 16 else (2 if m <= 32 else 4))
     need = (m + 1) // 2
     tw = 4 if need <= 4 else (8 if need <= 8 else 16)
-    return {"factor": "+".join("k_factor<float,%d>" % nt for nt in nts),
+    return {"factor": "+".join("k_factor<float,%d,%d>" % (nt, 1 if (nt in one_copy and nt == 4) else 2) for nt in nts),
             "spikes": "+".join("k_sweep<float,%d,%d,true>" % (nt, ct_vw(nt)) for nt in nts),
             "ysweep": "+".join("k_sweep<float,%d,%d,false>" % (nt, ct_y(nt)) for nt in nts),
             "recover": "+".join("k_recover<float,%d,%d>" % (nt, ct_y(nt)) for nt in nts),
@@ -511,7 +512,8 @@ def bench_batch(args, torch, hs):
             counts[st][0] += f
             counts[st][1] += b
     counts["spikes"] = spike_counts(eager["work"], 4)
-    names = kernel_names_for(sorted(nts), C4["nrhs_hi"])
+    # the b = 64 group of the batch has thousands of partitions: the one-copy factor variant
+    names = kernel_names_for(sorted(nts), C4["nrhs_hi"], one_copy=(4,))
     stage_roof = stage_roofline(avg, counts, passes, groups, "c4", names)
     roof = roofline_of(stage_roof, avg)
 

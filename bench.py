@@ -279,7 +279,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison runs")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a plain device copy of 256 MiB three times (calibrates the HBM byte counters)")
+    ap.add_argument("--probe-stream", action="store_true",
+                    help="only measure the streaming bandwidth plain torch kernels reach (read, copy, write)")
     args = ap.parse_args()
+    if args.probe_stream:
+        return probe_stream()
     if args.probe_copy:
         import torch
         x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
@@ -302,6 +306,38 @@ def main():
     if args.config == "c5":
         return bench_serve(args, hs)
     return bench_single(args, torch, hs)
+
+
+def probe_stream():
+    """Streaming bandwidth of plain torch kernels over 1, 4 and 16 GiB: a read-only reduction
+    (sum of FP32, max of int32), a copy (read + write) and a fill (write).  HIP events around 10
+    launches after one untimed launch; one JSON line per case.  It is the practical ceiling the
+    stage rates are read against (the roofline itself uses the published 8 TB/s)."""
+    import torch
+    dev = torch.device("cuda:0")
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    for gib in (1, 4, 16):
+        n = gib * 2 ** 30 // 4
+        x = torch.ones(n, dtype=torch.float32, device=dev)
+        xi = torch.ones(n, dtype=torch.int32, device=dev)
+        y = torch.empty_like(x)
+        for name, fn, nbytes in (("read (sum f32)", lambda: x.sum(), 4 * n), ("read (max i32)", lambda: xi.amax(), 4 * n),
+                                 ("copy f32 (read + write)", lambda: y.copy_(x), 8 * n), ("write (fill f32)", lambda: y.fill_(1.0), 4 * n)):
+            ms = timed(fn)
+            print(json.dumps({"probe": name, "gib": gib, "ms": round(ms, 4), "GB_s": round(nbytes / ms / 1e6, 1)}), flush=True)
+        del x, xi, y
+    return 0
 
 
 def bench_single(args, torch, hs):

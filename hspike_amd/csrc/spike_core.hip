@@ -514,7 +514,6 @@ int ct_max(int tcode, int NT) {
 int ct_for(int ncols, int ctmax) { return std::min(ctmax, ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 4)); }
 
 // wide groups (wide_group(): b = 256, complex FP64 at b = 128) sweep one column tile per item
-template <typename T, int NT> constexpr bool wide_kernels() { return NT == 16 || (sizeof(T) == 16 && NT == 8); }
 
 template <typename T, int NT, int CT, bool VW>
 void launch_sweep(int nitems, hipStream_t s, const SweepItem *items, const PartDesc *parts, const T *D, const T *C,

@@ -166,3 +166,47 @@ def test_oversubscribed_b64_factor_matches_alone(env, oracle_mod):
         torch.cuda.synchronize()
         assert torch.equal(xs, s["x"])
         solo.close()
+
+
+def test_narrow_coupling_blocks_after_wide_ones(env, oracle_mod):
+    """With FP32 factors the empty column groups of the coupling blocks B_j / C_j are neither
+    written nor read.  One Batch object first solves full-band systems (every group of its 16- and
+    64-row blocks holds values), then asymmetric narrow ones of the same block sizes in the same
+    arrays: (ku, kl) = (3, 12), (12, 3), (1, 7) at b = 16 and (40, 20), (20, 40) at b = 64.  Their
+    solutions must be bit-for-bit those of a fresh solver (stale values never reach them, and B
+    follows ku, C follows kl), and match the oracle's FP64 direct solve (<= 1e-8)."""
+    torch, hs = env
+    dev = torch.device("cuda:0")
+
+    def systems_for(shapes, seed):
+        out = []
+        for i, (n, ku, kl) in enumerate(shapes):
+            A = hs.gen_banded(n, ku, kl, fill=0.6, dom=(1.1, 1.1), diag_floor=1.0, seed=seed + i)
+            B = hs.gen_dense(n, 3, seed=seed + 50 + i)
+            out.append(dict(n=n, nrhs=3, A=A, B=B, rowptr=torch.from_numpy(A.rowptr).to(dev),
+                            colind=torch.from_numpy(A.colind).to(dev), aij=torch.from_numpy(A.aij).to(dev),
+                            b=torch.from_numpy(np.ascontiguousarray(B.T)).to(dev),
+                            x=torch.zeros((3, n), dtype=torch.float64, device=dev)))
+        return out
+
+    bt = hs.Batch("mixed", tol=1e-10)
+    wide = systems_for([(40000, 16, 16), (30000, 15, 16), (40000, 64, 64), (30000, 60, 64)], 8100)
+    rel, _ = bt.solve_dev(wide)
+    torch.cuda.synchronize()
+    assert rel.max() <= 1e-10
+    narrow = systems_for([(40000, 3, 12), (30000, 12, 3), (40000, 1, 7), (40000, 40, 20), (30000, 20, 40)], 8200)
+    rel, _ = bt.solve_dev(narrow)
+    torch.cuda.synchronize()
+    assert rel.max() <= 1e-10, rel.max()
+    bt.close()
+    for s in narrow:
+        X64 = oracle_mod.direct_solve(oracle_mod.CSR(s["A"].n, s["A"].rowptr, s["A"].colind, s["A"].aij), s["B"])
+        X = s["x"].cpu().numpy().T
+        assert np.linalg.norm(X - X64) / np.linalg.norm(X64) <= 1e-8
+        fresh = hs.Batch("mixed", tol=1e-10)
+        xs = dict(s)
+        xs["x"] = torch.zeros_like(s["x"])
+        fresh.solve_dev([xs])
+        torch.cuda.synchronize()
+        assert torch.equal(xs["x"], s["x"])
+        fresh.close()

@@ -92,3 +92,25 @@ def test_repeat_rhs_uses_resident_factors(hs):
     srv.close()
     assert not i1["resident_hit"] and i2["resident_hit"] and i1["key"] == i2["key"]
     np.testing.assert_array_equal(X1, X2)
+
+
+def test_large_requests_cross_staging_chunks(hs):
+    """Requests whose arrays span several 32 MiB pinned staging chunks of a worker, up and down
+    (values 105 MB, column indices 53 MB, RHS and solution 51 MB, real; a complex one of 140 MB of
+    values): the solution equals the same request solved alone through a fresh handle bit for bit,
+    and meets the residual bar."""
+    srv = hs.Server("mixed", tol=1e-10, workers=2, max_batch=2)
+    reqs = []
+    for i, (n, k, m, cplx) in enumerate(((400000, 16, 16, False), (350000, 12, 5, True))):
+        A = hs.gen_banded(n, k, k, fill=1.0, dom=(1.2, 1.2), diag_floor=1.0, seed=9300 + i, complex=cplx)
+        B = hs.gen_dense(n, m, seed=9400 + i, complex=cplx)
+        reqs.append((A, B, srv.submit(A, B)))
+    out = []
+    for A, B, t in reqs:
+        X, info = srv.wait(t)
+        assert info["status"] == 0, info
+        out.append((A, B, X))
+    srv.close()
+    for A, B, X in out:
+        assert hs.residual(A, X, B).max() <= 1e-10
+        np.testing.assert_array_equal(X, alone(hs, A, B, "mixed"))

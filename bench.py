@@ -395,6 +395,23 @@ def bench_single(args, torch, hs):
     P_auto = mixed["last"]["P"]
     rhs_s = args.steps * nrhs / mixed["wall"]
 
+    # the same mixed solve from host memory, as a caller of the reference's host entry points sees
+    # it: CSR and B copied in from pageable arrays, factor + solve, X copied back (PCIe-inclusive;
+    # never the bench value).  Best of three calls after one warm call.
+    hts = []
+    hsol = hs.Solver("mixed", tol=cfg["tol"], max_refine=8)
+    for r in range(4):
+        t0 = time.perf_counter()
+        hsol.set_matrix(A)
+        hsol.solve(B)
+        hts.append(time.perf_counter() - t0)
+    hsol.close()
+    t_host = min(hts[1:])
+    host_bytes = (n + 1) * 4 + A.nnz * 12 + 2 * n * nrhs * 8
+    host_path = {"rhs_per_s": round(nrhs / t_host, 2), "ms": round(t_host * 1e3, 3), "bytes_moved": host_bytes,
+                 "note": "CSR and B from pageable host memory, factor + solve, X back (Solver.set_matrix + solve)"}
+    log("host path: %.3f ms per call (%.1f MB moved)" % (t_host * 1e3, host_bytes / 1e6))
+
     out_fp64, x64 = None, None
     if not args.no_fp64:
         f64 = run("fp64", 0, max(3, args.steps // 2), 2, graph=True)
@@ -465,6 +482,7 @@ def bench_single(args, torch, hs):
         "fp64": out_fp64,
         "mixed_vs_fp64": round(rhs_s / out_fp64["rhs_per_s"], 3) if out_fp64 else None,
         "resident_factors_rhs_per_s": round(nrhs / (mixed["resident_ms"] * 1e-3), 2),
+        "host_path": host_path,
         "graph": {"replayed_steps": int(sum(x["graph_replayed"] for x in mixed["stats"])),
                   "eager_rhs_per_s": round(len(eager["lat"]) * nrhs / eager["wall"], 2),
                   "eager_ms_per_step": round(eager["wall"] / len(eager["lat"]) * 1e3, 3)},

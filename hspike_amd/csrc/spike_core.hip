@@ -140,6 +140,138 @@ int upload(DevBuf &d, const void *h, size_t bytes, hipStream_t s) {
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Copies between pageable host arrays (what callers hand over) and device memory through two
+// pinned buffers: the runtime would move pageable data through small staging copies run on the
+// compute units.  An upload copies chunk k into one buffer while the DMA of chunk k - 1 drains
+// from the other; a download starts the DMA of chunk k + 1 before copying chunk k out.  Chunk
+// copies on the host are split over four threads.  The _cols forms move ncols columns of `width`
+// bytes between a strided host panel (pitch >= width) and a contiguous device one.  Without
+// pinned buffers (allocation refused) the copies go to the runtime directly.  One stager serves one
+// thread at a time.
+struct HostStager {
+    static constexpr size_t CHUNK = (size_t)32 << 20;
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pend[2] = {false, false};          // a DMA reading buf[k] may still run
+    bool ok = false;
+    HostStager() {
+        ok = true;
+        for (int k = 0; k < 2 && ok; ++k)
+            ok = hipHostMalloc(&buf[k], CHUNK, hipHostMallocDefault) == hipSuccess &&
+                 hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) == hipSuccess;
+        if (!ok) (void)hipGetLastError();
+    }
+    ~HostStager() {
+        for (int k = 0; k < 2; ++k) {
+            if (ev[k]) { (void)hipEventSynchronize(ev[k]); (void)hipEventDestroy(ev[k]); }
+            if (buf[k]) (void)hipHostFree(buf[k]);
+        }
+    }
+    HostStager(const HostStager &) = delete;
+    HostStager &operator=(const HostStager &) = delete;
+    static void par_copy(void *dst, const void *src, size_t len) {
+        constexpr int NTH = 4;
+        if (len < ((size_t)4 << 20)) { memcpy(dst, src, len); return; }
+        const size_t part = (len / NTH + 63) & ~(size_t)63;
+        std::thread th[NTH - 1];
+        for (int t = 1; t < NTH; ++t) {
+            const size_t o = std::min(len, part * t), l = std::min(len, part * (t + 1)) - o;
+            th[t - 1] = std::thread([=] { memcpy((char *)dst + o, (const char *)src + o, l); });
+        }
+        memcpy(dst, src, std::min(len, part));
+        for (auto &x : th) x.join();
+    }
+    // bytes [off, off + len) of the column image <-> the strided host panel
+    static void pack(void *chunk, const void *src, size_t spitch, size_t width, size_t off, size_t len) {
+        if (spitch == width) { par_copy(chunk, (const char *)src + off, len); return; }
+        for (size_t o = off; o < off + len;) {
+            const size_t c = o / width, in = o % width, piece = std::min(width - in, off + len - o);
+            memcpy((char *)chunk + (o - off), (const char *)src + c * spitch + in, piece);
+            o += piece;
+        }
+    }
+    static void unpack(void *dst, size_t dpitch, const void *chunk, size_t width, size_t off, size_t len) {
+        if (dpitch == width) { par_copy((char *)dst + off, chunk, len); return; }
+        for (size_t o = off; o < off + len;) {
+            const size_t c = o / width, in = o % width, piece = std::min(width - in, off + len - o);
+            memcpy((char *)dst + c * dpitch + in, (const char *)chunk + (o - off), piece);
+            o += piece;
+        }
+    }
+    int h2d_cols(void *dst, const void *src, size_t spitch, size_t width, size_t ncols, hipStream_t st) {
+        if (!ok) {
+            HIPCK(hipMemcpy2DAsync(dst, width, src, spitch, width, ncols, hipMemcpyHostToDevice, st));
+            return 0;
+        }
+        const size_t total = width * ncols;
+        int k = 0;
+        for (size_t off = 0; off < total; off += CHUNK, k ^= 1) {
+            const size_t len = std::min(CHUNK, total - off);
+            if (pend[k]) HIPCK(hipEventSynchronize(ev[k]));
+            pack(buf[k], src, spitch, width, off, len);
+            HIPCK(hipMemcpyAsync((char *)dst + off, buf[k], len, hipMemcpyHostToDevice, st));
+            HIPCK(hipEventRecord(ev[k], st));
+            pend[k] = true;
+        }
+        return 0;
+    }
+    int h2d(void *dst, const void *src, size_t bytes, hipStream_t st) { return h2d_cols(dst, src, bytes, bytes, 1, st); }
+    // returns once the data is in dst
+    int d2h_cols(void *dst, size_t dpitch, const void *src, size_t width, size_t ncols, hipStream_t st) {
+        if (!ok) {
+            HIPCK(hipMemcpy2DAsync(dst, dpitch, src, width, width, ncols, hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+            return 0;
+        }
+        const size_t total = width * ncols;
+        int k = 0, wk = -1;
+        size_t woff = 0, wlen = 0;
+        for (size_t off = 0; off < total || wk >= 0; off += CHUNK, k ^= 1) {
+            int nk = -1;
+            size_t noff = 0, nlen = 0;
+            if (off < total) {              // buf[k] is free: its previous contents were copied out
+                nlen = std::min(CHUNK, total - off);
+                noff = off;
+                HIPCK(hipMemcpyAsync(buf[k], (const char *)src + off, nlen, hipMemcpyDeviceToHost, st));
+                HIPCK(hipEventRecord(ev[k], st));
+                nk = k;
+            }
+            if (wk >= 0) {
+                HIPCK(hipEventSynchronize(ev[wk]));
+                unpack(dst, dpitch, buf[wk], width, woff, wlen);
+            }
+            wk = nk; woff = noff; wlen = nlen;
+        }
+        pend[0] = pend[1] = false;          // every DMA of the stream so far has completed
+        return 0;
+    }
+    int d2h(void *dst, const void *src, size_t bytes, hipStream_t st) { return d2h_cols(dst, bytes, src, bytes, 1, st); }
+    int drain() {                           // uploads issued so far have completed
+        for (int k = 0; k < 2; ++k)
+            if (pend[k]) { HIPCK(hipEventSynchronize(ev[k])); pend[k] = false; }
+        return 0;
+    }
+};
+
+// Stagers lent to the host entry points (kept for the life of the process; one per concurrent call)
+struct StagerLease {
+    static std::mutex &mu() { static std::mutex m; return m; }
+    static std::vector<HostStager *> &pool() { static std::vector<HostStager *> p; return p; }
+    HostStager *s;
+    StagerLease() {
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            if (!pool().empty()) { s = pool().back(); pool().pop_back(); return; }
+        }
+        s = new HostStager();
+    }
+    ~StagerLease() {
+        std::lock_guard<std::mutex> lk(mu());
+        pool().push_back(s);
+    }
+    HostStager *operator->() { return s; }
+};
+
 template <typename T> T s_zero_host() {
     T z;
     memset(&z, 0, sizeof z);
@@ -1729,10 +1861,14 @@ int spike_set_matrix_host(spike_handle_t *h, int dtype, int n, int64_t nnz, cons
     if ((e = E.own_rowptr.ensure((size_t)(n + 1) * 4)) || (e = E.own_colind.ensure((size_t)nnz * 4)) ||
         (e = E.own_aij.ensure((size_t)nnz * hs)))
         return e;
-    HIPCK(hipMemcpy(E.own_rowptr.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
-    if (nnz) {
-        HIPCK(hipMemcpy(E.own_colind.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(E.own_aij.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
+    {
+        StagerLease sg;
+        if ((e = sg->h2d(E.own_rowptr.p, rowptr, (size_t)(n + 1) * 4, nullptr))) return e;
+        if (nnz && ((e = sg->h2d(E.own_colind.p, colind, (size_t)nnz * 4, nullptr)) ||
+                    (e = sg->h2d(E.own_aij.p, aij, (size_t)nnz * hs, nullptr))))
+            return e;
+        if ((e = sg->drain())) return e;
+        HIPCK(hipStreamSynchronize(nullptr));
     }
     return spike_set_matrix_dev(h, dtype, n, nnz, E.own_rowptr.as<int>(), E.own_colind.as<int>(), E.own_aij.p);
 }
@@ -1769,10 +1905,14 @@ int spike_solve_host(spike_handle_t *h, int nrhs, const void *b, int ldb, void *
     const size_t hs = h_bytes(E.dtype), bytes = (size_t)n * nrhs * hs;
     int e;
     if ((e = E.stage_b.ensure(bytes)) || (e = E.stage_x.ensure(bytes))) return e;
-    HIPCK(hipMemcpy2D(E.stage_b.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
+    StagerLease sg;
+    if ((e = sg->h2d_cols(E.stage_b.p, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, nullptr)) || (e = sg->drain())) return e;
+    HIPCK(hipStreamSynchronize(nullptr));
     const int rc = spike_solve_dev(h, nrhs, E.stage_b.p, n, E.stage_x.p, n);
-    if (rc == SPIKE_SUCCESS || rc == SPIKE_ERR_NOT_CONVERGED)
-        HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, E.stage_x.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
+    if (rc == SPIKE_SUCCESS || rc == SPIKE_ERR_NOT_CONVERGED) {
+        HIPCK(hipStreamSynchronize(E.stream));      // the solve may run on the handle's own stream
+        if ((e = sg->d2h_cols(x, (size_t)ldx * hs, E.stage_x.p, (size_t)n * hs, nrhs, nullptr))) return e;
+    }
     return rc;
 }
 

@@ -144,7 +144,8 @@ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // pinned buffers: the runtime would move pageable data through small staging copies run on the
 // compute units.  An upload copies chunk k into one buffer while the DMA of chunk k - 1 drains
 // from the other; a download starts the DMA of chunk k + 1 before copying chunk k out.  Chunk
-// copies on the host are split over four threads.  The _cols forms move ncols columns of `width`
+// copies on the host are split over `threads` threads (1 where several stagers copy at once, as
+// the server's workers do).  The _cols forms move ncols columns of `width`
 // bytes between a strided host panel (pitch >= width) and a contiguous device one.  Without
 // pinned buffers (allocation refused) the copies go to the runtime directly.  One stager serves one
 // thread at a time.
@@ -154,7 +155,8 @@ struct HostStager {
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool pend[2] = {false, false};          // a DMA reading buf[k] may still run
     bool ok = false;
-    HostStager() {
+    int threads = 4;
+    explicit HostStager(int copy_threads = 4) : threads(copy_threads) {
         ok = true;
         for (int k = 0; k < 2 && ok; ++k)
             ok = hipHostMalloc(&buf[k], CHUNK, hipHostMallocDefault) == hipSuccess &&
@@ -169,20 +171,20 @@ struct HostStager {
     }
     HostStager(const HostStager &) = delete;
     HostStager &operator=(const HostStager &) = delete;
-    static void par_copy(void *dst, const void *src, size_t len) {
-        constexpr int NTH = 4;
-        if (len < ((size_t)4 << 20)) { memcpy(dst, src, len); return; }
-        const size_t part = (len / NTH + 63) & ~(size_t)63;
-        std::thread th[NTH - 1];
-        for (int t = 1; t < NTH; ++t) {
+    void par_copy(void *dst, const void *src, size_t len) const {
+        const int nth = std::max(1, std::min(threads, 8));
+        if (nth == 1 || len < ((size_t)4 << 20)) { memcpy(dst, src, len); return; }
+        const size_t part = (len / nth + 63) & ~(size_t)63;
+        std::thread th[7];
+        for (int t = 1; t < nth; ++t) {
             const size_t o = std::min(len, part * t), l = std::min(len, part * (t + 1)) - o;
             th[t - 1] = std::thread([=] { memcpy((char *)dst + o, (const char *)src + o, l); });
         }
         memcpy(dst, src, std::min(len, part));
-        for (auto &x : th) x.join();
+        for (int t = 1; t < nth; ++t) th[t - 1].join();
     }
     // bytes [off, off + len) of the column image <-> the strided host panel
-    static void pack(void *chunk, const void *src, size_t spitch, size_t width, size_t off, size_t len) {
+    void pack(void *chunk, const void *src, size_t spitch, size_t width, size_t off, size_t len) const {
         if (spitch == width) { par_copy(chunk, (const char *)src + off, len); return; }
         for (size_t o = off; o < off + len;) {
             const size_t c = o / width, in = o % width, piece = std::min(width - in, off + len - o);
@@ -190,7 +192,7 @@ struct HostStager {
             o += piece;
         }
     }
-    static void unpack(void *dst, size_t dpitch, const void *chunk, size_t width, size_t off, size_t len) {
+    void unpack(void *dst, size_t dpitch, const void *chunk, size_t width, size_t off, size_t len) const {
         if (dpitch == width) { par_copy((char *)dst + off, chunk, len); return; }
         for (size_t o = off; o < off + len;) {
             const size_t c = o / width, in = o % width, piece = std::min(width - in, off + len - o);

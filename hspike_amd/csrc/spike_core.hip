@@ -2395,17 +2395,19 @@ int spike_band_solve_host(int dtype, int n, int64_t nnz, const int *rowptr, cons
         (e = da.ensure((size_t)std::max<int64_t>(nnz, 1) * hs)) || (e = dx.ensure(panel)) || (e = db.ensure(panel)) ||
         (e = ws.ensure(band_ws_bytes(dtype, n, ku, kl, nrhs))))
         return e;
-    HIPCK(hipMemcpy(drp.p, rowptr, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
-    if (nnz) {
-        HIPCK(hipMemcpy(dci.p, colind, (size_t)nnz * 4, hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(da.p, aij, (size_t)nnz * hs, hipMemcpyHostToDevice));
-    }
-    if (nrhs) HIPCK(hipMemcpy2D(db.p, (size_t)n * hs, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, hipMemcpyHostToDevice));
+    StagerLease sg;
+    if ((e = sg->h2d(drp.p, rowptr, (size_t)(n + 1) * 4, nullptr))) return e;
+    if (nnz && ((e = sg->h2d(dci.p, colind, (size_t)nnz * 4, nullptr)) || (e = sg->h2d(da.p, aij, (size_t)nnz * hs, nullptr))))
+        return e;
+    if (nrhs && (e = sg->h2d_cols(db.p, b, (size_t)ldb * hs, (size_t)n * hs, nrhs, nullptr))) return e;
+    if ((e = sg->drain())) return e;
+    HIPCK(hipStreamSynchronize(nullptr));
     Carve c(ws.p, ws.bytes);
     const BandWork w = band_carve(c, dtype, n, ku, kl, nrhs);
     std::vector<double> rl(std::max(nrhs, 1));
     if ((e = band_run(dtype, n, ku, kl, drp.as<int>(), dci.as<int>(), da.p, nrhs, db.p, n, dx.p, n, rl.data(), w, 0))) return e;
-    if (nrhs) HIPCK(hipMemcpy2D(x, (size_t)ldx * hs, dx.p, (size_t)n * hs, (size_t)n * hs, nrhs, hipMemcpyDeviceToHost));
+    HIPCK(hipStreamSynchronize(nullptr));
+    if (nrhs && (e = sg->d2h_cols(x, (size_t)ldx * hs, dx.p, (size_t)n * hs, nrhs, nullptr))) return e;
     if (rel) for (int q = 0; q < nrhs; ++q) rel[q] = rl[q];
     return SPIKE_SUCCESS;
 }

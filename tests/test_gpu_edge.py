@@ -259,3 +259,36 @@ def test_pivots_outside_the_pivot_panels(hs, oracle_mod, shift, k, prec):
     assert hs.residual(A, X, B).max() <= 1e-10
     Xo = oracle_mod.direct_solve(oracle_mod.CSR(A.n, A.rowptr, A.colind, A.aij), B)
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) <= 1e-9
+
+
+def test_host_panels_with_leading_dimensions(hs):
+    """spike_solve_host and spike_band_solve_host with leading dimensions above n (ldb = n + 7,
+    ldx = n + 3) on panels of 43 MB, wider than one 32 MiB pinned staging chunk, so chunk bounds
+    fall inside columns: the host panels are packed into and out of the staging buffers column by
+    column.  The solutions equal those of the ld = n calls bit for bit, and X's padding rows are
+    left as they were."""
+    L = hs.load()
+    vp = ctypes.c_void_p
+    p = lambda a: a.ctypes.data_as(vp)
+    A = hs.gen_banded(600000, 8, 8, fill=0.5, dom=(1.1, 1.1), diag_floor=1.0, seed=61)
+    n, m = A.n, 9
+    B = np.asfortranarray(hs.gen_dense(n, m, seed=62))
+    ldb, ldx = n + 7, n + 3
+    Bp = np.zeros((ldb, m), order="F")
+    Bp[:n] = B
+    s = hs.Solver("mixed", tol=1e-10)
+    s.set_matrix(A)
+    X0 = s.solve(B)
+    Xp = np.full((ldx, m), 7.0, order="F")
+    assert L.spike_solve_host(s._h, m, p(Bp), ldb, p(Xp), ldx) == 0
+    s.close()
+    np.testing.assert_array_equal(Xp[:n], X0)
+    assert np.all(Xp[n:] == 7.0)
+    assert hs.residual(A, X0, B).max() <= 1e-10
+    X1, _ = hs.band_solve(A, B)
+    rel = np.zeros(m)
+    Xq = np.full((ldx, m), 7.0, order="F")
+    assert L.spike_band_solve_host(hs.SPIKE_DTYPE_D, n, A.nnz, p(A.rowptr), p(A.colind), p(A.aij), m, p(Bp), ldb,
+                                   p(Xq), ldx, p(rel)) == 0
+    np.testing.assert_array_equal(Xq[:n], X1)
+    assert np.all(Xq[n:] == 7.0)

@@ -72,7 +72,7 @@ def test_batch_repeat_is_deterministic(env):
     first = [s["x"].clone() for s in systems]
     for s in systems:
         s["x"].zero_()
-    bt.solve_dev(systems)
+    bt.solve_dev(bt.prepare(systems))          # the same batch through its prepared descriptors
     torch.cuda.synchronize()
     for a, s in zip(first, systems):
         assert torch.equal(a, s["x"])

@@ -344,3 +344,23 @@ def test_memory_abi(L):
     assert cnt_f.value == f0 + 5
     assert not L._spike_malloc(16, 2 ** 62, 16, b"test", 2)   # overflow: NULL, no abort
     assert cnt_a.value == a0 + 4
+
+
+def test_prepared_batch_descriptors():
+    """Batch.prepare / PreparedBatch: one spike_system_t per system carrying its sizes and the
+    tensors' data pointers, the tensors kept referenced (CPU tensors stand in for device ones; no
+    library call is made)."""
+    import torch
+    import hspike_amd as hs
+    systems = []
+    for i in range(3):
+        n, nnz, m = 100 + i, 300 + i, 1 + i
+        systems.append(dict(n=n, nrhs=m, rowptr=torch.zeros(n + 1, dtype=torch.int32),
+                            colind=torch.zeros(nnz, dtype=torch.int32), aij=torch.zeros(nnz, dtype=torch.float64),
+                            b=torch.zeros((m, n), dtype=torch.float64), x=torch.zeros((m, n), dtype=torch.float64)))
+    pb = hs.PreparedBatch(systems)
+    assert pb.count == 3 and len(pb.systems) == 3
+    for a, s in zip(pb.arr, systems):
+        assert (a.n, a.nnz, a.nrhs) == (s["n"], s["colind"].numel(), s["nrhs"])
+        assert (a.d_rowptr, a.d_colind, a.d_aij, a.d_b, a.d_x) == tuple(
+            s[k].data_ptr() for k in ("rowptr", "colind", "aij", "b", "x"))

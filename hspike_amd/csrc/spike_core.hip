@@ -565,6 +565,7 @@ struct Engine {
         return 0;
     }
     std::vector<uint64_t> bw_key;           // systems described by d_bwchunks / d_bwsys
+    size_t bw_nchunks = 0;                  // row chunks in d_bwchunks
     DevBuf own_rowptr, own_colind, own_aij;   // host-matrix copies (spike_set_matrix_host)
     DevBuf stage_b, stage_x;                   // host solve staging
     spike_stats_t st;
@@ -734,27 +735,31 @@ int Engine::factor() {
 
     // ---- validation and bandwidth of every system (one launch)
     {
-        std::vector<RowChunk> ch;
-        std::vector<SysDev> sd(nsys);
-        for (int i = 0; i < nsys; ++i) {
-            memset(&sd[i], 0, sizeof(SysDev));
-            sd[i].rowptr = sys[i].d_rowptr;
-            sd[i].colind = sys[i].d_colind;
-            sd[i].n = sys[i].n;
-            for (int r = 0; r < sys[i].n; r += BW_ROWS) ch.push_back({i, r});
-        }
+        // the chunk and system descriptors of the scan are rebuilt only when the systems' arrays
+        // or sizes change (a batch factored again keeps them)
         std::vector<uint64_t> key;
+        key.reserve(3 * (size_t)nsys);
         for (int i = 0; i < nsys; ++i)
             key.insert(key.end(), {(uint64_t)(uintptr_t)sys[i].d_rowptr, (uint64_t)(uintptr_t)sys[i].d_colind,
                                    (uint64_t)sys[i].n});
         if (key != bw_key) {
+            std::vector<RowChunk> ch;
+            std::vector<SysDev> sd(nsys);
+            for (int i = 0; i < nsys; ++i) {
+                memset(&sd[i], 0, sizeof(SysDev));
+                sd[i].rowptr = sys[i].d_rowptr;
+                sd[i].colind = sys[i].d_colind;
+                sd[i].n = sys[i].n;
+                for (int r = 0; r < sys[i].n; r += BW_ROWS) ch.push_back({i, r});
+            }
             if ((e = upload(d_bwchunks, ch.data(), ch.size() * sizeof(RowChunk), s))) return e;
             if ((e = upload(d_bwsys, sd.data(), sd.size() * sizeof(SysDev), s))) return e;
             bw_key = key;
+            bw_nchunks = ch.size();
         }
         if ((e = d_bw.ensure(3 * sizeof(int) * nsys)) || (e = zero_ints(d_bw, 3 * (size_t)nsys))) return e;
-        if (!ch.empty())
-            hipLaunchKernelGGL(k_bandwidth, dim3((unsigned)ch.size()), dim3(256), 0, s, d_bwchunks.as<RowChunk>(),
+        if (bw_nchunks)
+            hipLaunchKernelGGL(k_bandwidth, dim3((unsigned)bw_nchunks), dim3(256), 0, s, d_bwchunks.as<RowChunk>(),
                                d_bwsys.as<SysDev>(), d_bw.as<int>());
         HIPCK(hipGetLastError());
         std::vector<int> bw(3 * nsys);

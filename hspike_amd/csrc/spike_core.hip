@@ -473,6 +473,7 @@ struct Group {
     int ct_vw = 4, ct_y = 1;                // column tiles per sweep workgroup (V/W items, RHS items)
     int ymax = 0;
     std::vector<uint64_t> solve_key;        // descriptors of the last solve chunk (reused when equal)
+    std::vector<uint64_t> plan_key;         // members, sizes, bandwidths, P and CSR arrays of the plan
     bool plan_uploaded = false;             // d_parts .. d_sysdev hold this plan
     bool vw_uploaded = false;               // items_vw holds this plan's V/W items
     int red_last = 0;                       // buffer (0: red0, 1: red1) holding the last iterate
@@ -801,6 +802,28 @@ int Engine::factor() {
         if (members.empty()) continue;
         if ((int)groups.size() <= ngroups) groups.emplace_back(new Group());
         Group &g = *groups[ngroups];
+        // a plan already uploaded for the same members (sizes, bandwidths, partition counts and
+        // CSR arrays) is kept as it is: a batch factored again skips the rebuild of its
+        // partition, interface and row-chunk lists
+        std::vector<uint64_t> pk;
+        pk.reserve(8 * members.size() + 2);
+        pk.push_back((uint64_t)nts[q]);
+        pk.push_back((uint64_t)tcode);
+        for (int i : members) {
+            const SysHost &S = sys[i];
+            pk.insert(pk.end(), {(uint64_t)i, (uint64_t)S.n, (uint64_t)S.P, (uint64_t)S.ku, (uint64_t)S.kl,
+                                 (uint64_t)(uintptr_t)S.d_rowptr, (uint64_t)(uintptr_t)S.d_colind,
+                                 (uint64_t)(uintptr_t)S.d_aij});
+        }
+        if (g.plan_uploaded && pk == g.plan_key) {
+            for (int gs = 0; gs < (int)members.size(); ++gs) {
+                sys[members[gs]].group = ngroups;
+                sys[members[gs]].gidx = gs;
+            }
+            ++ngroups;
+            continue;
+        }
+        g.plan_key = pk;
         std::vector<PartDesc> old_parts;
         old_parts.swap(g.parts);
         const bool same_type = g.NT == nts[q] && g.tcode == tcode && g.sys == members;
@@ -890,7 +913,8 @@ int Engine::factor() {
         if ((e = g.pivinfo.ensure(np * 2 * sizeof(double)))) return e;
         if ((e = g.status.ensure(4 * sizeof(int))) || (e = zero_ints(g.status, 4))) return e;
         // zero the coupling tiles, which the conversion scatters into (fill kernels; no memset
-        // nodes in code a caller may capture); D, U and C are written whole by the conversion
+        // nodes in code a caller may capture); the conversion writes D whole, and U and C whole
+        // except, with FP32 factors, the column groups no entry can reach (never read)
         T_SWITCH(tcode, {
             if ((e = fill_zero<T>(g.Bt, np * bb, s)) || (e = fill_zero<T>(g.Ct, np * bb, s))) return e;
         });
